@@ -1,0 +1,12 @@
+// Adjoint gradient of the log-likelihood.
+#pragma once
+
+#include "engine.hpp"
+
+namespace exa {
+
+// l(theta) and d l / d theta = 1/2 alpha^T dS alpha - 1/2 tr(S^-1 dS).
+double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric, double* grad,
+                           double* logdet_out, double* dotp_out, int* info);
+
+}  // namespace exa

@@ -1,0 +1,863 @@
+// The reference's C entry points for the exact-dense likelihood path,
+// implemented over the HIP engine.  Declarations and citations are in
+// include/exageostat_gpu.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/exageostat_gpu.h"
+#include "engine.hpp"
+#include "mixed.hpp"
+#include "adjoint.hpp"
+
+using namespace exa;
+
+namespace {
+
+thread_local int g_status = 0;
+thread_local std::string g_error;
+
+void set_error(int status, const std::string& msg) {
+  g_status = status;
+  g_error = msg;
+}
+
+// Library-wide streams and workspace for the descriptor-level calls.
+struct Runtime {
+  hipStream_t s1 = nullptr, s2 = nullptr;
+  CholWorkspace ws;
+  double* d_partial = nullptr;
+  long partial_len = 0;
+  Runtime() {
+    EXA_HIP(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+    EXA_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+  }
+  double* partial(long n) {
+    if (n > partial_len) {
+      if (d_partial) (void)hipFree(d_partial);
+      EXA_HIP(hipMalloc(&d_partial, sizeof(double) * n));
+      partial_len = n;
+    }
+    return d_partial;
+  }
+};
+Runtime* g_rt = nullptr;
+
+Runtime& rt() {
+  if (!g_rt) g_rt = new Runtime();
+  return *g_rt;
+}
+
+// Descriptor payload: a device tile matrix (owned or borrowed), or caller
+// host memory for scalar accumulators.
+struct DescImpl {
+  TileMatrix* tm = nullptr;
+  DeviceVector* vec = nullptr;  // column vectors (Z, Zcpy) are plain padded arrays
+  bool owns = false;
+  double* host = nullptr;
+  DeviceLocations rows, cols;  // location cache for dcmg on this matrix
+  ~DescImpl() {
+    if (owns) delete tm;
+  }
+};
+
+CHAM_desc_t* make_desc(TileMatrix* tm, bool owns) {
+  auto* d = new CHAM_desc_t();
+  auto* impl = new DescImpl();
+  impl->tm = tm;
+  impl->owns = owns;
+  d->mat = impl;
+  d->dtyp = ChamRealDouble;
+  d->mb = d->nb = tm->ts;
+  d->bsiz = tm->ts * tm->ts;
+  d->m = tm->m;
+  d->n = tm->n;
+  d->mt = tm->mt;
+  d->nt = tm->nt;
+  d->lm = tm->mt * tm->ts;
+  d->ln = tm->nt * tm->ts;
+  d->uplo = tm->lower ? ChamLower : ChamUpperLower;
+  return d;
+}
+
+CHAM_desc_t* make_vec_desc(DeviceVector* v, int ts) {
+  auto* d = new CHAM_desc_t();
+  auto* impl = new DescImpl();
+  impl->vec = v;
+  d->mat = impl;
+  d->dtyp = ChamRealDouble;
+  d->mb = ts;
+  d->nb = 1;
+  d->bsiz = ts;
+  d->m = (int)v->n;
+  d->n = 1;
+  d->mt = (int)(v->padded / ts);
+  d->nt = 1;
+  d->lm = (int)v->padded;
+  d->ln = 1;
+  d->uplo = ChamUpperLower;
+  return d;
+}
+
+CHAM_desc_t* make_user_desc(double* host, int m, int n) {
+  auto* d = new CHAM_desc_t();
+  auto* impl = new DescImpl();
+  impl->host = host;
+  d->mat = impl;
+  d->dtyp = ChamRealDouble;
+  d->mb = d->nb = 1;
+  d->bsiz = 1;
+  d->m = d->lm = m;
+  d->n = d->ln = n;
+  d->mt = m;
+  d->nt = n;
+  d->uplo = ChamUpperLower;
+  return d;
+}
+
+void free_desc(void*& p) {
+  if (!p) return;
+  auto* d = static_cast<CHAM_desc_t*>(p);
+  delete static_cast<DescImpl*>(d->mat);
+  delete d;
+  p = nullptr;
+}
+
+// Sequence = stream + host reductions to finish at wait time.
+struct SeqImpl {
+  hipStream_t s = nullptr;
+  bool owns = false;
+  struct Pending {
+    std::vector<double> h;
+    double* target;
+  };
+  std::vector<Pending> pending;
+};
+
+hipStream_t seq_stream(RUNTIME_sequence_t* seq) {
+  if (seq && seq->impl) return static_cast<SeqImpl*>(seq->impl)->s;
+  return rt().s1;
+}
+
+Context* ctx_of(MLE_data* data) {
+  if (!data || !data->gpu) throw std::runtime_error("MLE_data has no GPU context (call EXAGEOSTAT_dmle_Call)");
+  return static_cast<Context*>(data->gpu);
+}
+
+int metric_of(const char* dm) { return (dm && !strcmp(dm, "gc")) ? 1 : 0; }
+
+bool is_bivariate(const char* k) {
+  return k && (!strncmp(k, "bivariate_matern_parsimonious", 29));
+}
+
+void copy_vec_to_device(DeviceVector& v, const double* h, long n, hipStream_t s) {
+  EXA_HIP(hipMemsetAsync(v.d, 0, sizeof(double) * v.padded, s));
+  EXA_HIP(hipMemcpyAsync(v.d, h, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  EXA_HIP(hipStreamSynchronize(s));
+}
+
+void print_iteration(MLE_data* data, const double* theta, int np, double loglik) {
+  if (data->silent) return;
+  fprintf(stderr, " %3d- Model Parameters (", data->iter_count + 1);
+  for (int i = 0; i < np; ++i) fprintf(stderr, "%.8f%s", theta[i], i + 1 < np ? "," : "");
+  fprintf(stderr, ")----> LogLi: %.18f\n", loglik);
+  fprintf(stderr, " ---- Facto Time: %6.2f\n", data->time_facto);
+  fprintf(stderr, " ---- Matrix Generation Time: %6.2f\n", data->time_gen);
+  fprintf(stderr, " ---- Total Time: %6.2f\n", data->time_total);
+}
+
+// Morton order of locations_obs_zsort_inplace (misc/compute/MLE_misc.c:
+// 405-535): coordinates rescaled to [0, 2^32-1], keys interleave y above x
+// bit by bit; ties keep (even popcount) or reverse (odd) their input order,
+// which is what the reference's in-place partition sort produces.
+void obs_zsort(long n, double* x, double* y, double* z) {
+  if (n <= 1) return;
+  double mn[2] = {x[0], y[0]}, mx[2] = {x[0], y[0]};
+  for (long i = 1; i < n; ++i) {
+    mn[0] = std::min(mn[0], x[i]);
+    mx[0] = std::max(mx[0], x[i]);
+    mn[1] = std::min(mn[1], y[i]);
+    mx[1] = std::max(mx[1], y[i]);
+  }
+  struct Item {
+    uint64_t key;
+    long tie;
+    long idx;
+  };
+  std::vector<Item> it(n);
+  for (long i = 0; i < n; ++i) {
+    const uint32_t ux = (uint32_t)((x[i] - mn[0]) / (mx[0] - mn[0]) * 4294967295.0);
+    const uint32_t uy = (uint32_t)((y[i] - mn[1]) / (mx[1] - mn[1]) * 4294967295.0);
+    uint64_t key = 0;
+    for (int b = 31; b >= 0; --b) {
+      key = (key << 1) | ((uy >> b) & 1u);
+      key = (key << 1) | ((ux >> b) & 1u);
+    }
+    const int pop = __builtin_popcountll(key);
+    it[i] = {key, (pop % 2 == 0) ? i : -i, i};
+  }
+  std::sort(it.begin(), it.end(), [](const Item& a, const Item& b) {
+    return a.key != b.key ? a.key < b.key : a.tie < b.tie;
+  });
+  std::vector<double> nx(n), ny(n), nz(n);
+  for (long i = 0; i < n; ++i) {
+    nx[i] = x[it[i].idx];
+    ny[i] = y[it[i].idx];
+    nz[i] = z[it[i].idx];
+  }
+  std::copy(nx.begin(), nx.end(), x);
+  std::copy(ny.begin(), ny.end(), y);
+  std::copy(nz.begin(), nz.end(), z);
+}
+
+uint32_t part1by1(uint32_t v) {
+  v &= 0x0000ffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+uint32_t compact1by1(uint32_t v) {
+  v &= 0x55555555u;
+  v = (v | (v >> 1)) & 0x33333333u;
+  v = (v | (v >> 2)) & 0x0f0f0f0fu;
+  v = (v | (v >> 4)) & 0x00ff00ffu;
+  v = (v | (v >> 8)) & 0x0000ffffu;
+  return v;
+}
+
+template <class F>
+auto guarded(F&& f, decltype(f()) on_error) -> decltype(f()) {
+  try {
+    g_status = 0;
+    g_error.clear();
+    return f();
+  } catch (const std::invalid_argument& e) {
+    set_error(-1, e.what());
+  } catch (const std::exception& e) {
+    set_error(-2, e.what());
+  }
+  return on_error;
+}
+
+}  // namespace
+
+extern "C" {
+
+int exageostat_gpu_last_status(void) { return g_status; }
+const char* exageostat_gpu_last_error(void) { return g_error.c_str(); }
+
+int exageostat_gpu_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+void exageostat_init(int* ncores, int* gpus, int* dts, int* lts) {
+  (void)ncores;
+  (void)gpus;
+  (void)dts;
+  (void)lts;
+  guarded([&]() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) throw std::runtime_error("no HIP device");
+    EXA_HIP(hipSetDevice(0));
+    rt();
+    return 0;
+  }, 0);
+}
+
+void exageostat_finalize(void) {
+  guarded([&]() {
+    if (g_rt) EXA_HIP(hipDeviceSynchronize());
+    return 0;
+  }, 0);
+}
+
+// ------------------------------------------------------------ descriptors --
+int EXAGEOSTAT_Desc_Create(CHAM_desc_t** desc, CHAM_enum uplo, int mb, int m, int n) {
+  return guarded([&]() {
+    auto* tm = new TileMatrix();
+    try {
+      tm->allocate(m, n, mb, uplo == ChamLower);
+    } catch (...) {
+      delete tm;
+      throw;
+    }
+    *desc = make_desc(tm, true);
+    return 0;
+  }, -1);
+}
+
+int EXAGEOSTAT_Desc_Create_User(CHAM_desc_t** desc, double* host_mat, int m, int n) {
+  return guarded([&]() {
+    *desc = make_user_desc(host_mat, m, n);
+    return 0;
+  }, -1);
+}
+
+int EXAGEOSTAT_Desc_Destroy(CHAM_desc_t** desc) {
+  if (!desc || !*desc) return 0;
+  void* p = *desc;
+  free_desc(p);
+  *desc = nullptr;
+  return 0;
+}
+
+int EXAGEOSTAT_Sequence_Create(RUNTIME_sequence_t** sequence) {
+  return guarded([&]() {
+    auto* s = new RUNTIME_sequence_t();
+    auto* impl = new SeqImpl();
+    EXA_HIP(hipStreamCreateWithFlags(&impl->s, hipStreamNonBlocking));
+    impl->owns = true;
+    s->impl = impl;
+    s->status = 0;
+    *sequence = s;
+    return 0;
+  }, -1);
+}
+
+int EXAGEOSTAT_Sequence_Wait(RUNTIME_sequence_t* sequence) {
+  return guarded([&]() {
+    if (!sequence || !sequence->impl) return 0;
+    auto* impl = static_cast<SeqImpl*>(sequence->impl);
+    EXA_HIP(hipStreamSynchronize(impl->s));
+    for (auto& p : impl->pending) {
+      double acc = 0.0;
+      for (double v : p.h) acc += v;
+      *p.target += acc;
+    }
+    impl->pending.clear();
+    return sequence->status;
+  }, -1);
+}
+
+int EXAGEOSTAT_Sequence_Destroy(RUNTIME_sequence_t* sequence) {
+  if (!sequence) return 0;
+  auto* impl = static_cast<SeqImpl*>(sequence->impl);
+  if (impl) {
+    if (impl->owns && impl->s) (void)hipStreamDestroy(impl->s);
+    delete impl;
+  }
+  delete sequence;
+  return 0;
+}
+
+int EXAGEOSTAT_Lapack_to_Tile(const double* A, int lda, CHAM_desc_t* desc) {
+  return guarded([&]() {
+    auto* impl = static_cast<DescImpl*>(desc->mat);
+    if (impl->host) {
+      for (int j = 0; j < desc->n; ++j)
+        for (int i = 0; i < desc->m; ++i) impl->host[i + (long)j * desc->m] = A[i + (long)j * lda];
+      return 0;
+    }
+    if (impl->vec) {
+      copy_vec_to_device(*impl->vec, A, impl->vec->n, rt().s1);
+      return 0;
+    }
+    impl->tm->from_host(A, lda, rt().s1);
+    return 0;
+  }, -1);
+}
+
+int EXAGEOSTAT_Tile_to_Lapack(CHAM_desc_t* desc, double* A, int lda) {
+  return guarded([&]() {
+    auto* impl = static_cast<DescImpl*>(desc->mat);
+    if (impl->host) {
+      for (int j = 0; j < desc->n; ++j)
+        for (int i = 0; i < desc->m; ++i) A[i + (long)j * lda] = impl->host[i + (long)j * desc->m];
+      return 0;
+    }
+    if (impl->vec) {
+      EXA_HIP(hipMemcpyAsync(A, impl->vec->d, sizeof(double) * impl->vec->n, hipMemcpyDeviceToHost, rt().s1));
+      EXA_HIP(hipStreamSynchronize(rt().s1));
+      return 0;
+    }
+    impl->tm->to_host(A, lda, rt().s1);
+    return 0;
+  }, -1);
+}
+
+int EXAGEOSTAT_dpotrf_Tile(CHAM_enum uplo, CHAM_desc_t* A) {
+  return guarded([&]() {
+    if (uplo != ChamLower) throw std::invalid_argument("only ChamLower is supported");
+    auto* impl = static_cast<DescImpl*>(A->mat);
+    if (!impl->tm->lower) throw std::invalid_argument("descriptor must be lower-packed");
+    return tile_potrf(*impl->tm, rt().ws, rt().s1, rt().s2, true);
+  }, -1);
+}
+
+int EXAGEOSTAT_dtrsv_Tile(CHAM_enum trans, CHAM_desc_t* L, double* b) {
+  return guarded([&]() {
+    auto* impl = static_cast<DescImpl*>(L->mat);
+    TileMatrix& tm = *impl->tm;
+    DeviceVector v;
+    v.allocate(tm.m, (long)tm.mt * tm.ts);
+    copy_vec_to_device(v, b, tm.m, rt().s1);
+    tile_trsv(tm, rt().ws, v.d, trans == ChamTrans ? 1 : 0, rt().s1);
+    EXA_HIP(hipMemcpyAsync(b, v.d, sizeof(double) * tm.m, hipMemcpyDeviceToHost, rt().s1));
+    EXA_HIP(hipStreamSynchronize(rt().s1));
+    return 0;
+  }, -1);
+}
+
+// ------------------------------------------------------------- tile tasks --
+int EXAGEOSTAT_MLE_dcmg_Tile_Async(CHAM_enum uplo, CHAM_desc_t* descA, location* l1, location* l2,
+                                   location* lm, double* theta, char* dm, char* kernel_fun,
+                                   RUNTIME_sequence_t* sequence, RUNTIME_request_t* request) {
+  (void)lm;
+  (void)request;
+  (void)uplo;
+  return guarded([&]() {
+    if (sequence && sequence->status != 0) return -2;
+    const KernelId kid = kernel_from_name(kernel_fun);
+    if (kid == kUnknownKernel) throw std::invalid_argument(std::string("unsupported kernel: ") + (kernel_fun ? kernel_fun : "(null)"));
+    auto* impl = static_cast<DescImpl*>(descA->mat);
+    TileMatrix& tm = *impl->tm;
+    const long nr = kid == kBivariate ? tm.m / 2 : tm.m;
+    const long nc = kid == kBivariate ? tm.n / 2 : tm.n;
+    hipStream_t s = seq_stream(sequence);
+    impl->rows.upload(l1->x, l1->y, nr, s);
+    impl->cols.upload(l2->x, l2->y, nc, s);
+    generate_covariance(tm, kid, theta, impl->rows, impl->cols, metric_of(dm), s);
+    return 0;
+  }, -1);
+}
+
+int MORSE_MLE_dcmg_Tile_Async(CHAM_enum uplo, CHAM_desc_t* descA, location* l1, location* l2, location* lm,
+                              double* theta, char* dm, char* kernel_fun, RUNTIME_sequence_t* sequence,
+                              RUNTIME_request_t* request) {
+  return EXAGEOSTAT_MLE_dcmg_Tile_Async(uplo, descA, l1, l2, lm, theta, dm, kernel_fun, sequence, request);
+}
+
+int EXAGEOSTAT_MLE_dmdet_Tile_Async(CHAM_desc_t* descA, RUNTIME_sequence_t* sequence,
+                                    RUNTIME_request_t* request, CHAM_desc_t* descdet) {
+  (void)request;
+  return guarded([&]() {
+    if (sequence && sequence->status != 0) return -2;
+    auto* impl = static_cast<DescImpl*>(descA->mat);
+    auto* dimpl = static_cast<DescImpl*>(descdet->mat);
+    if (!dimpl->host) throw std::invalid_argument("descdet must be a host scalar descriptor");
+    TileMatrix& tm = *impl->tm;
+    hipStream_t s = seq_stream(sequence);
+    if (sequence && sequence->impl) {
+      auto* simpl = static_cast<SeqImpl*>(sequence->impl);
+      // Per-sequence partial buffer lives with the pending entry's host copy.
+      double* dpart = rt().partial(tm.mt);
+      launch_logdet(tm.view(), dpart, s);
+      SeqImpl::Pending p;
+      p.h.resize(tm.mt);
+      p.target = dimpl->host;
+      EXA_HIP(hipMemcpyAsync(p.h.data(), dpart, sizeof(double) * tm.mt, hipMemcpyDeviceToHost, s));
+      EXA_HIP(hipStreamSynchronize(s));
+      simpl->pending.push_back(std::move(p));
+    } else {
+      *dimpl->host += tile_logdet(tm, rt().partial(tm.mt), s);
+    }
+    return 0;
+  }, -1);
+}
+
+int MORSE_MLE_dmdet_Tile_Async(CHAM_desc_t* descA, RUNTIME_sequence_t* sequence, RUNTIME_request_t* request,
+                               CHAM_desc_t* descdet) {
+  return EXAGEOSTAT_MLE_dmdet_Tile_Async(descA, sequence, request, descdet);
+}
+
+// ------------------------------------------------------------ MLE driver --
+void EXAGEOSTAT_dmle_Call(MLE_data* data, int ncores, int gpus, int dts, int p_grid, int q_grid, int N,
+                          int nZobs, int nZmiss) {
+  (void)ncores;
+  (void)gpus;
+  (void)p_grid;
+  (void)q_grid;
+  guarded([&]() {
+    if (data->gpu) MLE_Finalize(data);
+    auto* ctx = new Context(N, dts, nZobs, nZmiss);
+    data->gpu = ctx;
+    data->descC = make_desc(&ctx->C, false);
+    data->descZ = make_vec_desc(&ctx->Z, dts);
+    data->descZcpy = make_vec_desc(&ctx->Zcpy, dts);
+    data->det = 0.0;
+    data->dotp = 0.0;
+    data->descdet = make_user_desc(&data->det, 1, 1);
+    data->descproduct = make_user_desc(&data->dotp, 1, 1);
+    data->iter_count = 0;
+    data->status = 0;
+    auto* seq = new RUNTIME_sequence_t();
+    auto* simpl = new SeqImpl();
+    simpl->s = ctx->s1;
+    simpl->owns = false;
+    seq->impl = simpl;
+    seq->status = 0;
+    data->sequence = seq;
+    data->request = new RUNTIME_request_t();
+    return 0;
+  }, 0);
+}
+
+void MLE_Finalize(MLE_data* data) {
+  guarded([&]() {
+    if (!data) return 0;
+    free_desc(data->descC);
+    free_desc(data->descZ);
+    free_desc(data->descZcpy);
+    free_desc(data->descdet);
+    free_desc(data->descproduct);
+    if (data->sequence) {
+      EXAGEOSTAT_Sequence_Destroy(static_cast<RUNTIME_sequence_t*>(data->sequence));
+      data->sequence = nullptr;
+    }
+    delete static_cast<RUNTIME_request_t*>(data->request);
+    data->request = nullptr;
+    delete static_cast<Context*>(data->gpu);
+    data->gpu = nullptr;
+    return 0;
+  }, 0);
+}
+
+void MLE_get_zobs(MLE_data* data, double* z, int n) {
+  guarded([&]() {
+    Context* ctx = ctx_of(data);
+    EXA_HIP(hipMemcpyAsync(z, ctx->Zcpy.d, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->s1));
+    EXA_HIP(hipStreamSynchronize(ctx->s1));
+    return 0;
+  }, 0);
+}
+
+location* GenerateXYLoc(int n, int seed) {
+  srand(seed);
+  auto* loc = static_cast<location*>(malloc(sizeof(location)));
+  loc->x = static_cast<double*>(malloc(sizeof(double) * n));
+  loc->y = static_cast<double*>(malloc(sizeof(double) * n));
+  loc->z = nullptr;
+  const int sqrtn = (int)ceil(sqrt((double)n));
+  int index = 0;
+  for (int i = 0; i < sqrtn && index < n; ++i)
+    for (int j = 0; j < sqrtn && index < n; ++j) {
+      // uniform_distribution(-0.4, 0.4), x drawn before y.
+      const double ux = (double)rand() / (1.0 + RAND_MAX) * 0.8 - 0.4;
+      loc->x[index] = ((i + 1) - 0.5 + ux) / sqrtn;
+      const double uy = (double)rand() / (1.0 + RAND_MAX) * 0.8 - 0.4;
+      loc->y[index] = ((j + 1) - 0.5 + uy) / sqrtn;
+      ++index;
+    }
+  // Morton sort with 16-bit quantisation (zsort_locations).
+  std::vector<uint32_t> code(n);
+  for (int i = 0; i < n; ++i) {
+    const uint16_t qx = (uint16_t)(loc->x[i] * 65535.0 + 0.5);
+    const uint16_t qy = (uint16_t)(loc->y[i] * 65535.0 + 0.5);
+    code[i] = (part1by1(qy) << 1) + part1by1(qx);
+  }
+  std::sort(code.begin(), code.end());
+  for (int i = 0; i < n; ++i) {
+    loc->x[i] = (double)compact1by1(code[i]) / 65535.0;
+    loc->y[i] = (double)compact1by1(code[i] >> 1) / 65535.0;
+  }
+  return loc;
+}
+
+void pick_random_points(MLE_data* data, double* Zobs, double* Zactual, int nZmiss, int nZobs, int N) {
+  guarded([&]() {
+    std::vector<double> Z(N), x(N), y(N);
+    MLE_get_zobs(data, Z.data(), N);
+    for (int i = 0; i < N; ++i) {
+      x[i] = data->l1.x[i];
+      y[i] = data->l1.y[i];
+    }
+    // shuffle (misc/compute/MLE_misc.c:1238-1259), same rand() stream.
+    if (N > 1) {
+      for (size_t i = 0; i < (size_t)N - 1; ++i) {
+        const size_t j = i + rand() / (RAND_MAX / ((size_t)N - i) + 1);
+        std::swap(Z[i], Z[j]);
+        std::swap(x[i], x[j]);
+        std::swap(y[i], y[j]);
+      }
+    }
+    for (int i = 0; i < nZobs; ++i) Zobs[i] = Z[nZmiss + i];
+    for (int i = 0; i < nZmiss; ++i) Zactual[i] = Z[i];
+    auto* mx = static_cast<double*>(malloc(sizeof(double) * (nZmiss > 0 ? nZmiss : 1)));
+    auto* my = static_cast<double*>(malloc(sizeof(double) * (nZmiss > 0 ? nZmiss : 1)));
+    auto* ox = static_cast<double*>(malloc(sizeof(double) * (nZobs > 0 ? nZobs : 1)));
+    auto* oy = static_cast<double*>(malloc(sizeof(double) * (nZobs > 0 ? nZobs : 1)));
+    std::copy(x.begin(), x.begin() + nZmiss, mx);
+    std::copy(y.begin(), y.begin() + nZmiss, my);
+    std::copy(x.begin() + nZmiss, x.begin() + nZmiss + nZobs, ox);
+    std::copy(y.begin() + nZmiss, y.begin() + nZmiss + nZobs, oy);
+    obs_zsort(nZobs, ox, oy, Zobs);
+    obs_zsort(nZmiss, mx, my, Zactual);
+    data->lmiss.x = mx;
+    data->lmiss.y = my;
+    data->lmiss.z = nullptr;
+    data->lobs.x = ox;
+    data->lobs.y = oy;
+    data->lobs.z = nullptr;
+    return 0;
+  }, 0);
+}
+
+void EXAGEOSTAT_MLE_dzvg_Tile(MLE_data* data, double* Nrand, double* initial_theta, int n, int dts, int log) {
+  (void)dts;
+  (void)log;
+  guarded([&]() {
+    Context* ctx = ctx_of(data);
+    const KernelId kid = kernel_from_name(data->kernel_fun);
+    if (kid == kUnknownKernel) throw std::invalid_argument("unsupported kernel");
+    const long nloc = kid == kBivariate ? n / 2 : n;
+    ctx->l1.upload(data->l1.x, data->l1.y, nloc, ctx->s1);
+    generate_covariance(ctx->C, kid, initial_theta, ctx->l1, ctx->l1, metric_of(data->dm), ctx->s1);
+    const int info = tile_potrf(ctx->C, ctx->ws, ctx->s1, ctx->s2, true);
+    if (info != 0) throw std::runtime_error("dzvg: covariance is not positive definite");
+    copy_vec_to_device(ctx->work, Nrand, n, ctx->s1);
+    ctx->gemv_part.allocate((long)ctx->C.mt * ctx->C.mt * ctx->C.ts, (long)ctx->C.mt * ctx->C.mt * ctx->C.ts);
+    EXA_HIP(hipMemsetAsync(ctx->Z.d, 0, sizeof(double) * ctx->Z.padded, ctx->s1));
+    launch_gemv_full(ctx->C.view(), ctx->work.d, ctx->Z.d, ctx->gemv_part.d, ctx->s1);
+    EXA_HIP(hipMemcpyAsync(ctx->Zcpy.d, ctx->Z.d, sizeof(double) * ctx->Z.padded, hipMemcpyDeviceToDevice,
+                           ctx->s1));
+    EXA_HIP(hipStreamSynchronize(ctx->s1));
+    ctx->gemv_part.release();
+    data->iter_count = 0;
+    return 0;
+  }, 0);
+}
+
+void MLE_zvg(MLE_data* data, double* Nrand, double* initial_theta, int n, int ts, int log, int p_grid,
+             int q_grid) {
+  (void)p_grid;
+  (void)q_grid;
+  data->iter_count = 0;
+  EXAGEOSTAT_MLE_dzvg_Tile(data, Nrand, initial_theta, n, ts, log);
+}
+
+double EXAGEOSTAT_dmle_Tile(unsigned n, const double* theta, double* grad, void* vdata) {
+  auto* data = static_cast<MLE_data*>(vdata);
+  const double out = guarded([&]() {
+    Context* ctx = ctx_of(data);
+    const KernelId kid = kernel_from_name(data->kernel_fun);
+    if (kid == kUnknownKernel) throw std::invalid_argument(std::string("unsupported kernel: ") + (data->kernel_fun ? data->kernel_fun : "(null)"));
+    const long nloc = kid == kBivariate ? ctx->N / 2 : ctx->N;
+    ctx->l1.upload(data->l1.x, data->l1.y, nloc, ctx->s1);
+    if (data->iter_count == 0) {
+      // Keep a pristine copy of Z for later iterations (MLE_dexact.c:303-317).
+      EXA_HIP(hipMemcpyAsync(ctx->Zcpy.d, ctx->Z.d, sizeof(double) * ctx->Z.padded, hipMemcpyDeviceToDevice,
+                             ctx->s1));
+    }
+    double logdet = 0.0, dotp = 0.0;
+    int info = 0;
+    double loglik;
+    if (grad) {
+      loglik = adjoint_loglik_grad(*ctx, kid, theta, metric_of(data->dm), grad, &logdet, &dotp, &info);
+    } else {
+      loglik = ctx->loglik(kid, theta, metric_of(data->dm), &logdet, &dotp, &info);
+    }
+    data->det = 0.5 * logdet;
+    data->dotp = dotp;
+    data->logdet = logdet;
+    data->status = info;
+    data->time_gen = ctx->last.gen;
+    data->time_facto = ctx->last.facto;
+    data->time_logdet = ctx->last.logdet;
+    data->time_solve = ctx->last.solve;
+    data->time_total = ctx->last.total;
+    if (info != 0) {
+      set_error(info, "covariance matrix is not positive definite");
+      loglik = -HUGE_VAL;
+    }
+    const int np = n ? (int)n : kernel_num_params(kid);
+    print_iteration(data, theta, np, loglik);
+    data->variance = theta[0];
+    data->iter_count++;
+    data->avg_exec_time_per_iter += ctx->last.facto + ctx->last.logdet + ctx->last.solve;
+    const double flops = (double)ctx->N * (((1.0 / 6.0) * ctx->N + 0.5) * ctx->N + 1.0 / 3.0) +
+                         (double)ctx->N * (((1.0 / 6.0) * ctx->N) * ctx->N - 1.0 / 6.0) +
+                         (double)ctx->N * (ctx->N + 1.0);
+    data->avg_flops_per_iter += flops / 1e9 / (ctx->last.facto + ctx->last.solve);
+    data->final_loglik = loglik;
+    if (info != 0) g_status = info;
+    return loglik;
+  }, -HUGE_VAL);
+  if (g_status < 0 && data) data->status = g_status;
+  return out;
+}
+
+double MORSE_dmle_Tile(unsigned n, const double* theta, double* grad, void* data) {
+  return EXAGEOSTAT_dmle_Tile(n, theta, grad, data);
+}
+
+double EXAGEOSTAT_sdmle_Tile(unsigned n, const double* theta, double* grad, void* vdata) {
+  auto* data = static_cast<MLE_data*>(vdata);
+  const double out = guarded([&]() {
+    Context* ctx = ctx_of(data);
+    const KernelId kid = kernel_from_name(data->kernel_fun);
+    if (kid == kUnknownKernel) throw std::invalid_argument("unsupported kernel");
+    const long nloc = kid == kBivariate ? ctx->N / 2 : ctx->N;
+    ctx->l1.upload(data->l1.x, data->l1.y, nloc, ctx->s1);
+    if (data->iter_count == 0) {
+      EXA_HIP(hipMemcpyAsync(ctx->Zcpy.d, ctx->Z.d, sizeof(double) * ctx->Z.padded, hipMemcpyDeviceToDevice,
+                             ctx->s1));
+    }
+    MixedOptions opt;
+    opt.fp64_corr = data->mp_fp64_corr;
+    opt.fp32_corr = data->mp_fp32_corr;
+    opt.bf16 = data->mp_bf16 != 0;
+    opt.ir_steps = data->mp_ir_steps;
+    opt.diag_thick = data->diag_thick;
+    double logdet = 0.0, dotp = 0.0;
+    int info = 0;
+    double loglik = mixed_loglik(*ctx, kid, theta, metric_of(data->dm), opt, &logdet, &dotp, &info);
+    if (grad) {
+      for (unsigned i = 0; i < n; ++i) grad[i] = std::nan("");
+    }
+    data->det = 0.5 * logdet;
+    data->dotp = dotp;
+    data->logdet = logdet;
+    data->status = info;
+    data->time_gen = ctx->last.gen;
+    data->time_facto = ctx->last.facto;
+    data->time_logdet = ctx->last.logdet;
+    data->time_solve = ctx->last.solve;
+    data->time_total = ctx->last.total;
+    if (info != 0) {
+      set_error(info, "covariance matrix is not positive definite");
+      loglik = -HUGE_VAL;
+    }
+    print_iteration(data, theta, n ? (int)n : kernel_num_params(kid), loglik);
+    data->iter_count++;
+    data->final_loglik = loglik;
+    if (info != 0) g_status = info;
+    return loglik;
+  }, -HUGE_VAL);
+  if (g_status < 0 && data) data->status = g_status;
+  return out;
+}
+
+double MLE_alg(unsigned n, const double* theta, double* grad, void* vdata) {
+  auto* data = static_cast<MLE_data*>(vdata);
+  if (data->computation && strcmp(data->computation, "exact") != 0) {
+    set_error(-1, "only computation=\"exact\" is provided");
+    data->status = -1;
+    return -HUGE_VAL;
+  }
+  if (data->precision == 2) return EXAGEOSTAT_sdmle_Tile(n, theta, grad, vdata);
+  if (data->precision != 0) {
+    set_error(-1, "precision must be 0 (double) or 2 (mixed)");
+    data->status = -1;
+    return -HUGE_VAL;
+  }
+  return EXAGEOSTAT_dmle_Tile(n, theta, grad, vdata);
+}
+
+// ------------------------------------------------------------- prediction --
+void EXAGEOSTAT_dmle_Predict_Allocate(MLE_data* data, int nZmiss, int nZobs, int dts, int p_grid, int q_grid,
+                                      int mse_flag) {
+  (void)p_grid;
+  (void)q_grid;
+  (void)mse_flag;
+  guarded([&]() {
+    if (nZmiss <= 0) throw std::invalid_argument("Number of missing values should be positive value");
+    Context* ctx = ctx_of(data);
+    if (is_bivariate(data->kernel_fun)) {
+      nZobs *= 2;
+      nZmiss *= 2;
+    }
+    ctx->C22.allocate(nZobs, nZobs, dts, true);
+    ctx->C12.allocate(nZmiss, nZobs, dts, false);
+    ctx->Zobs.allocate(nZobs, (long)ctx->C22.mt * dts);
+    ctx->Zmiss.allocate(nZmiss, (long)ctx->C12.mt * dts);
+    ctx->Zactual.allocate(nZmiss, (long)ctx->C12.mt * dts);
+    ctx->gemv_part.allocate((long)ctx->C12.nt * ctx->C12.mt * dts, (long)ctx->C12.nt * ctx->C12.mt * dts);
+    return 0;
+  }, 0);
+}
+
+void prediction_init(MLE_data* data, int nZmiss, int nZobs, int ts, int p_grid, int q_grid, int mse_flag) {
+  EXAGEOSTAT_dmle_Predict_Allocate(data, nZmiss, nZobs, ts, p_grid, q_grid, mse_flag);
+}
+
+void prediction_finalize(MLE_data* data) {
+  guarded([&]() {
+    if (!data || !data->gpu) return 0;
+    Context* ctx = ctx_of(data);
+    ctx->C22.release();
+    ctx->C12.release();
+    ctx->Zobs.release();
+    ctx->Zmiss.release();
+    ctx->Zactual.release();
+    ctx->gemv_part.release();
+    return 0;
+  }, 0);
+}
+
+double EXAGEOSTAT_dmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
+                                    double* Zactual, double* Zmiss, int n) {
+  (void)n;
+  if (nZmiss <= 0) {
+    fprintf(stderr, " Number of missing values should be positive value\n");
+    return -1;
+  }
+  return guarded([&]() {
+    Context* ctx = ctx_of(data);
+    const KernelId kid = kernel_from_name(data->kernel_fun);
+    if (kid == kUnknownKernel) throw std::invalid_argument("unsupported kernel");
+    if (!ctx->C22.allocated() || ctx->C22.m != nZobs || ctx->C12.m != nZmiss)
+      EXAGEOSTAT_dmle_Predict_Allocate(data, kid == kBivariate ? nZmiss / 2 : nZmiss,
+                                       kid == kBivariate ? nZobs / 2 : nZobs, ctx->ts, 1, 1, 1);
+    if (g_status < 0) throw std::runtime_error(g_error);
+    const bool biv = kid == kBivariate;
+    const long nobs_loc = biv ? nZobs / 2 : nZobs;
+    const long nmiss_loc = biv ? nZmiss / 2 : nZmiss;
+    hipStream_t s = ctx->s1;
+    ctx->lobs.upload(data->lobs.x, data->lobs.y, nobs_loc, s);
+    ctx->lmiss.upload(data->lmiss.x, data->lmiss.y, nmiss_loc, s);
+    const double t0 = wall_seconds();
+    generate_covariance(ctx->C22, kid, theta, ctx->lobs, ctx->lobs, metric_of(data->dm), s);
+    generate_covariance(ctx->C12, kid, theta, ctx->lmiss, ctx->lobs, metric_of(data->dm), s);
+    copy_vec_to_device(ctx->Zobs, Zobs, nZobs, s);
+    const int info = tile_potrf(ctx->C22, ctx->ws, s, ctx->s2, true);
+    if (info != 0) throw std::runtime_error("prediction: C22 is not positive definite");
+    tile_trsv(ctx->C22, ctx->ws, ctx->Zobs.d, 0, s);
+    tile_trsv(ctx->C22, ctx->ws, ctx->Zobs.d, 1, s);
+    launch_gemv_full(ctx->C12.view(), ctx->Zobs.d, ctx->Zmiss.d, ctx->gemv_part.d, s);
+    EXA_HIP(hipMemcpyAsync(Zmiss, ctx->Zmiss.d, sizeof(double) * nZmiss, hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipStreamSynchronize(s));
+    const double t1 = wall_seconds();
+    ctx->last.total = t1 - t0;
+    data->time_total = t1 - t0;
+    data->mserror = 0.0;
+    data->mserror1 = 0.0;
+    data->mserror2 = 0.0;
+    if (!Zactual) {
+      data->mserror = -1;
+      return -1.0;
+    }
+    double se = 0.0, se1 = 0.0, se2 = 0.0;
+    for (int i = 0; i < nZmiss; ++i) {
+      const double e = (Zactual[i] - Zmiss[i]) * (Zactual[i] - Zmiss[i]);
+      se += e;
+      if (i % 2 == 0) se1 += e; else se2 += e;
+    }
+    data->mserror = se / nZmiss;
+    if (biv) {
+      data->mserror1 = se1 / (nZmiss / 2);
+      data->mserror2 = se2 / (nZmiss / 2);
+    }
+    return data->mserror;
+  }, -1.0);
+}
+
+double MORSE_dmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
+                               double* Zactual, double* Zmiss, int n) {
+  return EXAGEOSTAT_dmle_Predict_Tile(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, n);
+}
+
+}  // extern "C"

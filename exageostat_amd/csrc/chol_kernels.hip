@@ -1,0 +1,245 @@
+// FP64 tile Cholesky kernels: diagonal-tile factorisation, panel triangular
+// solve and trailing update, all on FP64 MFMA.
+//
+// Replaces Chameleon's CHAMELEON_dpotrf_Tile(ChamLower, C) as called from
+// EXAGEOSTAT_dmle_Tile (exageostat_exact/src/compute/MLE_dexact.c:372);
+// Chameleon is not vendored, the algorithm is the right-looking tile
+// Cholesky (potrf of A(k,k), trsm of A(m,k), syrk/gemm of A(m,n), m>=n>k).
+#include "common.hpp"
+#include "kernels.hpp"
+#include "mfma.hpp"
+
+namespace exa {
+
+using C64 = GemmCfg<64, 64, 2, 2, 16>;
+
+// ---------------------------------------------------------------------------
+// Unblocked Cholesky of the 64x64 diagonal block j of a diagonal tile, in LDS,
+// plus the explicit inverse of the resulting lower-triangular block.
+__global__ void __launch_bounds__(256) k_potrf_block(double* T, int ts, int j, double* dinv, int* info,
+                                                     long col_base) {
+  constexpr int LD = 65;
+  __shared__ double L[64 * LD];
+  __shared__ double Inv[64 * LD];
+  __shared__ double rdiag[64];
+  const int tid = threadIdx.x;
+  const int r = tid & 63;
+  const int g = tid >> 6;
+  double* blk = T + (long)j * kNB * (ts + 1);
+  for (int c = g; c < 64; c += 4) L[c * LD + r] = blk[r + (long)c * ts];
+  __syncthreads();
+  for (int c = 0; c < 64; ++c) {
+    double d = L[c * LD + c];
+    const bool bad = !(d > 0.0);
+    d = bad ? __builtin_nan("") : sqrt(d);
+    if (bad && tid == 0) atomicCAS(info, 0, (int)(col_base + c + 1));
+    __syncthreads();
+    if (tid < 64) {
+      if (tid > c) L[c * LD + tid] /= d;
+      if (tid == c) L[c * LD + c] = d;
+    }
+    __syncthreads();
+    for (int cc = c + 1 + g; cc < 64; cc += 4) {
+      if (r >= cc) L[cc * LD + r] -= L[c * LD + r] * L[c * LD + cc];
+    }
+    __syncthreads();
+  }
+  if (tid < 64) rdiag[tid] = 1.0 / L[tid * LD + tid];
+  __syncthreads();
+  if (tid < 64) {
+    const int c = tid;
+    for (int rr = 0; rr < c; ++rr) Inv[c * LD + rr] = 0.0;
+    Inv[c * LD + c] = rdiag[c];
+    for (int rr = c + 1; rr < 64; ++rr) {
+      double s = 0.0;
+      for (int k = c; k < rr; ++k) s += L[k * LD + rr] * Inv[c * LD + k];
+      Inv[c * LD + rr] = -s * rdiag[rr];
+    }
+  }
+  __syncthreads();
+  for (int c = g; c < 64; c += 4) {
+    blk[r + (long)c * ts] = (r >= c) ? L[c * LD + r] : 0.0;
+    dinv[r + c * 64] = Inv[c * LD + r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// X L^T = B for one 64-row slab X (in place of B), L lower with nL diagonal
+// 64-blocks whose inverses are in dinv.  Column sweep: for block jb,
+// X_jb = (B_jb - X_<jb L_jb,<jb^T) inv(L_jb,jb)^T.
+__device__ void trsm_slab(double* X, long ldx, const double* L, long ldl, int nL, const double* dinv,
+                          double* lds) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave / C64::WGN;
+  const int wn = wave - wm * C64::WGN;
+  constexpr int LDT = 68;
+  for (int jb = 0; jb < nL; ++jb) {
+    dx4 acc[C64::FM][C64::FN];
+    double* Xj = X + (long)jb * kNB * ldx;
+    acc_load<C64>(acc, Xj, ldx);
+    gemm_nt_acc<C64>(acc, X, ldx, L + jb * kNB, ldl, jb * kNB, lds, -1.0);
+    double* Ts = lds;
+    double* Is = lds + 64 * LDT;
+#pragma unroll
+    for (int i = 0; i < C64::FM; ++i)
+#pragma unroll
+      for (int jj = 0; jj < C64::FN; ++jj) {
+        const int c = wn * C64::WN + jj * 16 + frag_col(lane);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = wm * C64::WM + i * 16 + frag_row(lane, v);
+          Ts[c * LDT + r] = acc[i][jj][v];
+        }
+      }
+    const double* inv = dinv + (long)jb * 4096;
+    for (int e = tid; e < 64 * 32; e += 256) {
+      const int kk = e >> 5;
+      const int c2 = e & 31;
+      *reinterpret_cast<double2*>(Is + kk * LDT + 2 * c2) =
+          *reinterpret_cast<const double2*>(inv + 2 * c2 + kk * 64);
+    }
+    __syncthreads();
+    dx4 x[C64::FM][C64::FN];
+    zero_acc<C64>(x);
+#pragma unroll 4
+    for (int kk = 0; kk < 64; kk += 4) {
+      const int krow = kk + (lane >> 4);
+      double a[C64::FM], b[C64::FN];
+#pragma unroll
+      for (int i = 0; i < C64::FM; ++i) a[i] = Ts[krow * LDT + wm * C64::WM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int jj = 0; jj < C64::FN; ++jj) b[jj] = Is[krow * LDT + wn * C64::WN + jj * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < C64::FM; ++i)
+#pragma unroll
+        for (int jj = 0; jj < C64::FN; ++jj) x[i][jj] = mfma_f64(a[i], b[jj], x[i][jj]);
+    }
+    acc_update<C64>(x, Xj, ldx, 0.0);
+    // X_jb is read back (from global memory) by the next column block.
+    __threadfence();
+    __syncthreads();
+  }
+}
+
+// Panel: slabs over tiles m in (k, mt) and their 64-row blocks.
+__global__ void __launch_bounds__(256) k_trsm_panel(TileView v, int k, const double* dinv) {
+  __shared__ __attribute__((aligned(16))) double lds[2 * 64 * 68];
+  const int q = v.ts / kNB;
+  const int s = blockIdx.x;
+  const int m = k + 1 + s / q;
+  const int rb = s - (s / q) * q;
+  double* X = tile_f64(v, m, k) + rb * kNB;
+  const double* L = tile_f64(v, k, k);
+  trsm_slab(X, v.ts, L, v.ts, q, dinv, lds);
+}
+
+// Inside a diagonal tile after block j is factored: blocks (i, j), i > j.
+__global__ void __launch_bounds__(256) k_trsm_intra(double* T, int ts, int j, const double* dinv_j) {
+  __shared__ __attribute__((aligned(16))) double lds[2 * 64 * 68];
+  const int i = j + 1 + blockIdx.x;
+  double* X = T + (long)i * kNB + (long)j * kNB * ts;
+  trsm_slab(X, ts, nullptr, ts, 1, dinv_j, lds);
+}
+
+// Inside a diagonal tile: blocks (i, l), j < l <= i: B_il -= B_ij B_lj^T.
+__global__ void __launch_bounds__(256) k_syrk_intra(double* T, int ts, int j) {
+  __shared__ __attribute__((aligned(16))) double lds[C64::LDS_DOUBLES];
+  int r, c;
+  tri_decode(blockIdx.x, r, c);
+  const int i = j + 1 + r;
+  const int l = j + 1 + c;
+  const double* A = T + (long)i * kNB + (long)j * kNB * ts;
+  const double* B = T + (long)l * kNB + (long)j * kNB * ts;
+  double* Cp = T + (long)i * kNB + (long)l * kNB * ts;
+  dx4 acc[C64::FM][C64::FN];
+  zero_acc<C64>(acc);
+  gemm_nt_acc<C64>(acc, A, ts, B, ts, kNB, lds, -1.0);
+  acc_update<C64>(acc, Cp, ts, 1.0);
+}
+
+// ---------------------------------------------------------------------------
+// Trailing update A(m,n) -= A(m,k) A(n,k)^T, one work-group per BM x BN block.
+// col_lo == col_hi: the single tile column col_lo (tiles m >= col_lo);
+// otherwise the whole trailing triangle from column col_lo.
+template <class C>
+__global__ void __launch_bounds__(C::NT) k_tile_update(TileView v, int k, int col_lo, int col_hi) {
+  __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
+  const int qm = v.ts / C::BM;
+  const int qn = v.ts / C::BN;
+  const long b = blockIdx.x;
+  const long p = b / (qm * qn);
+  const int rem = (int)(b - p * (qm * qn));
+  const int bi = rem % qm;
+  const int bj = rem / qm;
+  int m, n;
+  if (col_lo == col_hi) {
+    n = col_lo;
+    m = col_lo + (int)p;
+  } else {
+    int r, c;
+    tri_decode(p, r, c);
+    m = col_lo + r;
+    n = col_lo + c;
+  }
+  if (m == n && (bi + 1) * C::BM <= bj * C::BN) return;
+  const double* A = tile_f64(v, m, k) + bi * C::BM;
+  const double* B = tile_f64(v, n, k) + bj * C::BN;
+  double* Cp = tile_f64(v, m, n) + bi * C::BM + (long)bj * C::BN * v.ts;
+  dx4 acc[C::FM][C::FN];
+  zero_acc<C>(acc);
+  gemm_nt_acc<C>(acc, A, v.ts, B, v.ts, v.ts, lds, -1.0);
+  acc_update<C>(acc, Cp, v.ts, 1.0);
+}
+
+// ---------------------------------------------------------------------------
+void launch_potrf_tile(double* T, int ts, double* dinv_k, int* info, long col_base, hipStream_t s) {
+  const int q = ts / kNB;
+  for (int j = 0; j < q; ++j) {
+    hipLaunchKernelGGL(k_potrf_block, dim3(1), dim3(256), 0, s, T, ts, j, dinv_k + (long)j * 4096, info,
+                       col_base + (long)j * kNB);
+    if (j + 1 < q) {
+      const int nb = q - j - 1;
+      hipLaunchKernelGGL(k_trsm_intra, dim3(nb), dim3(256), 0, s, T, ts, j, dinv_k + (long)j * 4096);
+      hipLaunchKernelGGL(k_syrk_intra, dim3(nb * (nb + 1) / 2), dim3(256), 0, s, T, ts, j);
+    }
+  }
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_trsm_panel(const TileView& v, int k, const double* dinv_k, hipStream_t s) {
+  const int q = v.ts / kNB;
+  const long slabs = (long)(v.mt - k - 1) * q;
+  if (slabs <= 0) return;
+  hipLaunchKernelGGL(k_trsm_panel, dim3((unsigned)slabs), dim3(256), 0, s, v, k, dinv_k);
+  EXA_HIP(hipGetLastError());
+}
+
+template <class C>
+static void launch_update_cfg(const TileView& v, int k, int col_lo, int col_hi, hipStream_t s) {
+  long pairs;
+  if (col_lo == col_hi) {
+    pairs = v.mt - col_lo;
+  } else {
+    const long M = v.mt - col_lo;
+    pairs = M * (M + 1) / 2;
+  }
+  if (pairs <= 0) return;
+  const long blocks = pairs * (v.ts / C::BM) * (v.ts / C::BN);
+  hipLaunchKernelGGL(k_tile_update<C>, dim3((unsigned)blocks), dim3(C::NT), 0, s, v, k, col_lo, col_hi);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_tile_update(const TileView& v, int k, int col_lo, int col_hi, hipStream_t s) {
+  if (col_lo >= v.mt || col_hi < col_lo) return;
+  if (v.ts % 192 == 0) {
+    launch_update_cfg<GemmCfg<192, 192, 2, 4, 16>>(v, k, col_lo, col_hi, s);
+  } else if (v.ts % 128 == 0) {
+    launch_update_cfg<GemmCfg<128, 128, 2, 2, 16>>(v, k, col_lo, col_hi, s);
+  } else {
+    launch_update_cfg<C64>(v, k, col_lo, col_hi, s);
+  }
+}
+
+}  // namespace exa

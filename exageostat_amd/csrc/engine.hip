@@ -1,0 +1,405 @@
+// Host engine: device tile matrices, covariance generation dispatch, the
+// single-GPU tile Cholesky scheduler and the likelihood driver.
+#include "engine.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace exa {
+
+double wall_seconds() {
+  using clk = std::chrono::steady_clock;
+  return std::chrono::duration<double>(clk::now().time_since_epoch()).count();
+}
+
+KernelId kernel_from_name(const char* name) {
+  if (!name) return kUnknownKernel;
+  if (!strcmp(name, "univariate_matern_stationary")) return kUnivariate;
+  if (!strcmp(name, "univariate_matern_nuggets_stationary")) return kNugget;
+  if (!strcmp(name, "bivariate_matern_parsimonious") || !strcmp(name, "bivariate_matern_parsimonious_profile"))
+    return kBivariate;
+  return kUnknownKernel;
+}
+
+int kernel_num_params(KernelId k) {
+  switch (k) {
+    case kUnivariate: return 3;
+    case kNugget: return 4;
+    case kBivariate: return 6;
+    default: return 0;
+  }
+}
+
+static size_t prec_size(uint8_t p) { return p == kF64 ? 8 : (p == kF32 ? 4 : 2); }
+
+// ------------------------------------------------------------- TileMatrix --
+TileMatrix::~TileMatrix() { release(); }
+
+void TileMatrix::release() {
+  if (pool_) (void)hipFree(pool_);
+  if (d_ptr_) (void)hipFree(d_ptr_);
+  if (d_prec_) (void)hipFree(d_prec_);
+  pool_ = nullptr;
+  d_ptr_ = nullptr;
+  d_prec_ = nullptr;
+  h_ptr_.clear();
+  h_prec_.clear();
+  bytes_ = 0;
+}
+
+void TileMatrix::allocate(int m_, int n_, int ts_, bool lower_, const std::vector<uint8_t>& prec) {
+  if (ts_ <= 0 || ts_ % kNB != 0 || ts_ > 1024)
+    throw std::invalid_argument("tile size must be a multiple of 64 and at most 1024");
+  if (lower_ && m_ != n_) throw std::invalid_argument("lower-packed matrices must be square");
+  release();
+  m = m_;
+  n = n_;
+  ts = ts_;
+  lower = lower_;
+  mt = (m + ts - 1) / ts;
+  nt = (n + ts - 1) / ts;
+  const long ntiles = lower ? (long)mt * (mt + 1) / 2 : (long)mt * nt;
+  h_prec_.assign(ntiles, (uint8_t)kF64);
+  if (!prec.empty()) {
+    if ((long)prec.size() != ntiles) throw std::invalid_argument("precision map size mismatch");
+    h_prec_ = prec;
+  }
+  std::vector<size_t> off(ntiles);
+  size_t total = 0;
+  for (long t = 0; t < ntiles; ++t) {
+    off[t] = total;
+    total += ((size_t)ts * ts * prec_size(h_prec_[t]) + 255) & ~(size_t)255;
+  }
+  EXA_HIP(hipMalloc(&pool_, total));
+  bytes_ = total;
+  h_ptr_.resize(ntiles);
+  for (long t = 0; t < ntiles; ++t) h_ptr_[t] = static_cast<char*>(pool_) + off[t];
+  EXA_HIP(hipMalloc(&d_ptr_, sizeof(void*) * ntiles));
+  EXA_HIP(hipMemcpy(d_ptr_, h_ptr_.data(), sizeof(void*) * ntiles, hipMemcpyHostToDevice));
+  EXA_HIP(hipMalloc(&d_prec_, ntiles));
+  EXA_HIP(hipMemcpy(d_prec_, h_prec_.data(), ntiles, hipMemcpyHostToDevice));
+}
+
+TileView TileMatrix::view() const {
+  TileView v;
+  v.ptr = d_ptr_;
+  v.prec = d_prec_;
+  v.m = m;
+  v.n = n;
+  v.mt = mt;
+  v.nt = nt;
+  v.ts = ts;
+  v.lower = lower ? 1 : 0;
+  return v;
+}
+
+void* TileMatrix::tile_raw(int mm, int nn) const {
+  TileView v = view();
+  return h_ptr_[tile_index(v, mm, nn)];
+}
+
+double* TileMatrix::tile(int mm, int nn) const { return static_cast<double*>(tile_raw(mm, nn)); }
+
+uint8_t TileMatrix::prec(int mm, int nn) const {
+  TileView v = view();
+  return h_prec_[tile_index(v, mm, nn)];
+}
+
+void TileMatrix::to_host(double* A, long ld, hipStream_t s) const {
+  std::vector<double> buf((size_t)ts * ts);
+  std::vector<float> fbuf((size_t)ts * ts);
+  std::vector<uint16_t> hbuf((size_t)ts * ts);
+  for (int tn = 0; tn < nt; ++tn) {
+    for (int tm = lower ? tn : 0; tm < mt; ++tm) {
+      const uint8_t p = prec(tm, tn);
+      if (p == kF64) {
+        EXA_HIP(hipMemcpyAsync(buf.data(), tile_raw(tm, tn), buf.size() * 8, hipMemcpyDeviceToHost, s));
+      } else if (p == kF32) {
+        EXA_HIP(hipMemcpyAsync(fbuf.data(), tile_raw(tm, tn), fbuf.size() * 4, hipMemcpyDeviceToHost, s));
+      } else {
+        EXA_HIP(hipMemcpyAsync(hbuf.data(), tile_raw(tm, tn), hbuf.size() * 2, hipMemcpyDeviceToHost, s));
+      }
+      EXA_HIP(hipStreamSynchronize(s));
+      for (int j = 0; j < ts && (long)tn * ts + j < n; ++j)
+        for (int i = 0; i < ts && (long)tm * ts + i < m; ++i) {
+          double val;
+          const size_t e = i + (size_t)j * ts;
+          if (p == kF64) {
+            val = buf[e];
+          } else if (p == kF32) {
+            val = fbuf[e];
+          } else {
+            uint32_t u = (uint32_t)hbuf[e] << 16;
+            float f;
+            memcpy(&f, &u, 4);
+            val = f;
+          }
+          A[((long)tm * ts + i) + ((long)tn * ts + j) * ld] = val;
+        }
+    }
+  }
+}
+
+void TileMatrix::from_host(const double* A, long ld, hipStream_t s) {
+  std::vector<double> buf((size_t)ts * ts);
+  for (int tn = 0; tn < nt; ++tn) {
+    for (int tm = lower ? tn : 0; tm < mt; ++tm) {
+      if (prec(tm, tn) != kF64) throw std::invalid_argument("from_host supports FP64 tiles only");
+      for (int j = 0; j < ts; ++j)
+        for (int i = 0; i < ts; ++i) {
+          const long gi = (long)tm * ts + i, gj = (long)tn * ts + j;
+          buf[i + (size_t)j * ts] = (gi < m && gj < n) ? A[gi + gj * ld] : (gi == gj ? 1.0 : 0.0);
+        }
+      EXA_HIP(hipMemcpyAsync(tile_raw(tm, tn), buf.data(), buf.size() * 8, hipMemcpyHostToDevice, s));
+      EXA_HIP(hipStreamSynchronize(s));
+    }
+  }
+}
+
+// ---------------------------------------------------------- small helpers --
+void DeviceVector::allocate(long n_, long padded_) {
+  if (d && padded == padded_) {
+    n = n_;
+    return;
+  }
+  release();
+  n = n_;
+  padded = padded_;
+  EXA_HIP(hipMalloc(&d, sizeof(double) * padded));
+  EXA_HIP(hipMemset(d, 0, sizeof(double) * padded));
+}
+
+void DeviceVector::release() {
+  if (d) (void)hipFree(d);
+  d = nullptr;
+  n = padded = 0;
+}
+
+void DeviceLocations::upload(const double* x, const double* y, long n_, hipStream_t s) {
+  if (x == hx && y == hy && n_ == n && dx) return;
+  release();
+  n = n_;
+  hx = x;
+  hy = y;
+  EXA_HIP(hipMalloc(&dx, sizeof(double) * (n > 0 ? n : 1)));
+  EXA_HIP(hipMalloc(&dy, sizeof(double) * (n > 0 ? n : 1)));
+  EXA_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  EXA_HIP(hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  EXA_HIP(hipStreamSynchronize(s));
+}
+
+void DeviceLocations::release() {
+  if (dx) (void)hipFree(dx);
+  if (dy) (void)hipFree(dy);
+  dx = dy = nullptr;
+  hx = hy = nullptr;
+  n = 0;
+}
+
+// ------------------------------------------------------------- generation --
+void generate_covariance(TileMatrix& A, KernelId kid, const double* theta, const DeviceLocations& rows,
+                         const DeviceLocations& cols, int metric, hipStream_t s) {
+  TileView v = A.view();
+  const long ntiles = A.num_tiles();
+  if (kid == kUnivariate || kid == kNugget) {
+    MaternParams P;
+    P.x1 = rows.dx;
+    P.y1 = rows.dy;
+    P.x2 = cols.dx;
+    P.y2 = cols.dy;
+    P.metric = metric;
+    P.sigma2 = theta[0];
+    P.beta = theta[1];
+    P.nugget = (kid == kNugget) ? theta[3] : 0.0;
+    P.con = 1.0 / (std::pow(2.0, theta[2] - 1.0) * std::tgamma(theta[2]));
+    knu_prepare(theta[2], P.ko);
+    launch_gen_matern(v, P, 0, ntiles, s);
+  } else if (kid == kBivariate) {
+    if (A.ts % 2 != 0) throw std::invalid_argument("bivariate kernel needs an even tile size");
+    BivariateParams P;
+    P.x1 = rows.dx;
+    P.y1 = rows.dy;
+    P.x2 = cols.dx;
+    P.y2 = cols.dy;
+    P.metric = metric;
+    const double s1 = theta[0], s2 = theta[1], nu1 = theta[3], nu2 = theta[4], rho = theta[5];
+    P.beta = theta[2];
+    const double nu12 = 0.5 * (nu1 + nu2);
+    const double rho_t = rho * std::sqrt((std::tgamma(nu1 + 1) * std::tgamma(nu2 + 1)) /
+                                         (std::tgamma(nu1) * std::tgamma(nu2))) *
+                         std::tgamma(nu12) / std::tgamma(nu12 + 1);
+    P.sigma11 = s1;
+    P.sigma22 = s2;
+    P.sigma12 = rho_t * std::sqrt(s1 * s2);
+    P.con11 = 1.0 / (std::pow(2.0, nu1 - 1.0) * std::tgamma(nu1));
+    P.con22 = 1.0 / (std::pow(2.0, nu2 - 1.0) * std::tgamma(nu2));
+    P.con12 = 1.0 / (std::pow(2.0, nu12 - 1.0) * std::tgamma(nu12));
+    knu_prepare(nu1, P.ko11);
+    knu_prepare(nu2, P.ko22);
+    knu_prepare(nu12, P.ko12);
+    launch_gen_bivariate(v, P, 0, ntiles, s);
+  } else {
+    throw std::invalid_argument("unsupported kernel");
+  }
+}
+
+// --------------------------------------------------------------- Cholesky --
+CholWorkspace::~CholWorkspace() {
+  if (dinv_) (void)hipFree(dinv_);
+  if (info_) (void)hipFree(info_);
+  if (h_info_) (void)hipHostFree(h_info_);
+  for (auto e : ev_) (void)hipEventDestroy(e);
+}
+
+void CholWorkspace::ensure(int mt, int ts) {
+  if (dinv_ && mt <= mt_ && ts == ts_) return;
+  if (dinv_) (void)hipFree(dinv_);
+  dinv_ = nullptr;
+  mt_ = mt;
+  ts_ = ts;
+  EXA_HIP(hipMalloc(&dinv_, sizeof(double) * (size_t)mt * (ts / kNB) * 4096));
+  if (!info_) EXA_HIP(hipMalloc(&info_, sizeof(int)));
+  if (!h_info_) EXA_HIP(hipHostMalloc(&h_info_, sizeof(int), hipHostMallocDefault));
+}
+
+hipEvent_t CholWorkspace::ev(int i) {
+  while ((int)ev_.size() <= i) {
+    hipEvent_t e;
+    EXA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ev_.push_back(e);
+  }
+  return ev_[i];
+}
+
+int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2, bool sync) {
+  const int mt = A.mt;
+  const int ts = A.ts;
+  ws.ensure(mt, ts);
+  int* d_info = ws.info();
+  EXA_HIP(hipMemsetAsync(d_info, 0, sizeof(int), s1));
+  TileView v = A.view();
+  EXA_HIP(hipEventRecord(ws.ev(0), s1));
+  EXA_HIP(hipStreamWaitEvent(s2, ws.ev(0), 0));
+  launch_potrf_tile(A.tile(0, 0), ts, ws.dinv(0), d_info, 0, s1);
+  launch_trsm_panel(v, 0, ws.dinv(0), s1);
+  EXA_HIP(hipEventRecord(ws.ev(1), s1));
+  for (int k = 0; k + 1 < mt; ++k) {
+    hipEvent_t ev_panel_k = ws.ev(1 + 2 * k);
+    hipEvent_t ev_panel_k1 = ws.ev(1 + 2 * (k + 1));
+    hipEvent_t ev_rest_k = ws.ev(2 + 2 * k);
+    // Critical path: column k+1 by step k, then factor panel k+1.
+    launch_tile_update(v, k, k + 1, k + 1, s1);
+    launch_potrf_tile(A.tile(k + 1, k + 1), ts, ws.dinv(k + 1), d_info, (long)(k + 1) * ts, s1);
+    launch_trsm_panel(v, k + 1, ws.dinv(k + 1), s1);
+    EXA_HIP(hipEventRecord(ev_panel_k1, s1));
+    // Remaining trailing update of step k overlaps with the panel above.
+    if (k + 2 < mt) {
+      EXA_HIP(hipStreamWaitEvent(s2, ev_panel_k, 0));
+      launch_tile_update(v, k, k + 2, mt - 1, s2);
+    }
+    EXA_HIP(hipEventRecord(ev_rest_k, s2));
+    EXA_HIP(hipStreamWaitEvent(s1, ev_rest_k, 0));
+  }
+  // s2 has no work after the last rest update; make s2 wait for s1 so later
+  // work on either stream sees the finished factor.
+  EXA_HIP(hipEventRecord(ws.ev(2 * mt + 2), s1));
+  EXA_HIP(hipStreamWaitEvent(s2, ws.ev(2 * mt + 2), 0));
+  launch_zero_upper_diag(v, s1);
+  EXA_HIP(hipMemcpyAsync(ws.host_info(), d_info, sizeof(int), hipMemcpyDeviceToHost, s1));
+  if (!sync) return 0;
+  EXA_HIP(hipStreamSynchronize(s1));
+  return *ws.host_info();
+}
+
+void tile_trsv(const TileMatrix& L, const CholWorkspace& ws, double* z, int trans, hipStream_t s) {
+  TileView v = L.view();
+  if (!trans) {
+    for (int k = 0; k < L.mt; ++k) {
+      launch_trsv_tile(L.tile(k, k), L.ts, ws.dinv(k), z + (long)k * L.ts, 0, s);
+      launch_gemv_panel(v, k, z, s);
+    }
+  } else {
+    for (int k = L.mt - 1; k >= 0; --k) {
+      launch_trsv_tile(L.tile(k, k), L.ts, ws.dinv(k), z + (long)k * L.ts, 1, s);
+      launch_gemv_t_row(v, k, z, s);
+    }
+  }
+}
+
+double tile_logdet(const TileMatrix& L, double* d_partial, hipStream_t s) {
+  launch_logdet(L.view(), d_partial, s);
+  std::vector<double> h(L.mt);
+  EXA_HIP(hipMemcpyAsync(h.data(), d_partial, sizeof(double) * L.mt, hipMemcpyDeviceToHost, s));
+  EXA_HIP(hipStreamSynchronize(s));
+  double acc = 0.0;
+  for (double x : h) acc += x;
+  return acc;
+}
+
+double device_dot(const double* a, const double* b, long n, double* d_partial, hipStream_t s) {
+  const long nb = (n + 4095) / 4096;
+  launch_dot(a, b, n, d_partial, s);
+  std::vector<double> h(nb);
+  EXA_HIP(hipMemcpyAsync(h.data(), d_partial, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  EXA_HIP(hipStreamSynchronize(s));
+  double acc = 0.0;
+  for (double x : h) acc += x;
+  return acc;
+}
+
+// ---------------------------------------------------------------- Context --
+Context::Context(int N_, int ts_, int nZobs, int nZmiss) : N(N_), ts(ts_) {
+  (void)nZobs;
+  (void)nZmiss;
+  EXA_HIP(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
+  EXA_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+  C.allocate(N, N, ts, true);
+  const long padded = (long)C.mt * ts;
+  Z.allocate(N, padded);
+  Zcpy.allocate(N, padded);
+  work.allocate(N, padded);
+  ws.ensure(C.mt, ts);
+  ensure_partial(padded / 4096 + C.mt + 16);
+}
+
+Context::~Context() {
+  if (d_partial) (void)hipFree(d_partial);
+  if (s1) (void)hipStreamDestroy(s1);
+  if (s2) (void)hipStreamDestroy(s2);
+}
+
+void Context::ensure_partial(long n) {
+  if (n <= partial_len) return;
+  if (d_partial) (void)hipFree(d_partial);
+  EXA_HIP(hipMalloc(&d_partial, sizeof(double) * n));
+  partial_len = n;
+}
+
+double Context::loglik(KernelId kid, const double* theta, int metric, double* logdet_out, double* dotp_out,
+                       int* info) {
+  const double t0 = wall_seconds();
+  EXA_HIP(hipMemcpyAsync(work.d, Zcpy.d, sizeof(double) * work.padded, hipMemcpyDeviceToDevice, s1));
+  generate_covariance(C, kid, theta, l1, l1, metric, s1);
+  EXA_HIP(hipStreamSynchronize(s1));
+  const double t1 = wall_seconds();
+  const int inf = tile_potrf(C, ws, s1, s2, true);
+  const double t2 = wall_seconds();
+  if (info) *info = inf;
+  const double logdet = 2.0 * tile_logdet(C, d_partial, s1);
+  const double t3 = wall_seconds();
+  tile_trsv(C, ws, work.d, 0, s1);
+  const double dotp = device_dot(work.d, work.d, N, d_partial, s1);
+  const double t4 = wall_seconds();
+  last.gen = t1 - t0;
+  last.facto = t2 - t1;
+  last.logdet = t3 - t2;
+  last.solve = t4 - t3;
+  last.total = t4 - t0;
+  if (logdet_out) *logdet_out = logdet;
+  if (dotp_out) *dotp_out = dotp;
+  return -0.5 * dotp - 0.5 * logdet - (N / 2.0) * std::log(2.0 * M_PI);
+}
+
+}  // namespace exa

@@ -1,0 +1,144 @@
+// Host-side engine: device tile matrices, the single-GPU tile Cholesky
+// scheduler and the likelihood / prediction drivers.  Everything here is
+// C++; the C entry points of the reference live in capi.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace exa {
+
+// Kernel families selected by the reference's kernel strings
+// (runtime/starpu/codelets/codelet_zcmg.c:262-310).
+enum KernelId {
+  kUnivariate = 0,   // "univariate_matern_stationary"
+  kBivariate = 3,    // "bivariate_matern_parsimonious"
+  kNugget = 4,       // "univariate_matern_nuggets_stationary"
+  kUnknownKernel = -1,
+};
+KernelId kernel_from_name(const char* name);
+int kernel_num_params(KernelId k);
+
+// A tiled matrix in device memory.  Tiles are ts x ts, column-major; all
+// padding rows/columns are part of the tiles.
+class TileMatrix {
+ public:
+  TileMatrix() = default;
+  ~TileMatrix();
+  TileMatrix(const TileMatrix&) = delete;
+  TileMatrix& operator=(const TileMatrix&) = delete;
+
+  // prec: per-tile precision map (empty: all FP64), indexed like tile_index.
+  void allocate(int m, int n, int ts, bool lower, const std::vector<uint8_t>& prec = {});
+  void release();
+  bool allocated() const { return pool_ != nullptr; }
+  TileView view() const;
+  long num_tiles() const { return (long)h_ptr_.size(); }
+  double* tile(int m, int n) const;  // FP64 tiles only
+  void* tile_raw(int m, int n) const;
+  uint8_t prec(int m, int n) const;
+  size_t bytes() const { return bytes_; }
+  // Host <-> device copies of the logical m x n matrix (column-major, ld).
+  void to_host(double* A, long ld, hipStream_t s) const;
+  void from_host(const double* A, long ld, hipStream_t s);
+
+  int m = 0, n = 0, ts = 0, mt = 0, nt = 0;
+  bool lower = false;
+
+ private:
+  void* pool_ = nullptr;
+  size_t bytes_ = 0;
+  std::vector<void*> h_ptr_;
+  std::vector<uint8_t> h_prec_;
+  void** d_ptr_ = nullptr;
+  uint8_t* d_prec_ = nullptr;
+};
+
+// A device vector padded to mt*ts entries.
+struct DeviceVector {
+  double* d = nullptr;
+  long n = 0, padded = 0;
+  void allocate(long n_, long padded_);
+  void release();
+  ~DeviceVector() { release(); }
+};
+
+struct PhaseTimes {
+  double gen = 0, facto = 0, logdet = 0, solve = 0, total = 0;
+};
+
+// Device copy of a location set (x, y), cached by host pointer.
+struct DeviceLocations {
+  const double* hx = nullptr;
+  const double* hy = nullptr;
+  long n = 0;
+  double* dx = nullptr;
+  double* dy = nullptr;
+  void upload(const double* x, const double* y, long n_, hipStream_t s);
+  void release();
+  ~DeviceLocations() { release(); }
+};
+
+// Covariance generation for a kernel family and theta.
+void generate_covariance(TileMatrix& A, KernelId kid, const double* theta, const DeviceLocations& rows,
+                         const DeviceLocations& cols, int metric, hipStream_t s);
+
+// Workspace of the tile Cholesky (inverses of 64x64 diagonal blocks, info).
+class CholWorkspace {
+ public:
+  ~CholWorkspace();
+  void ensure(int mt, int ts);
+  double* dinv(int k) const { return dinv_ + (long)k * (ts_ / kNB) * 4096; }
+  int* info() const { return info_; }      // device
+  int* host_info() const { return h_info_; }  // pinned host mirror
+  hipEvent_t ev(int i);
+  int mt_ = 0, ts_ = 0;
+
+ private:
+  double* dinv_ = nullptr;
+  int* info_ = nullptr;
+  int* h_info_ = nullptr;
+  std::vector<hipEvent_t> ev_;
+};
+
+// Right-looking tile Cholesky with one-step look-ahead on two streams.
+// Returns 0 or the first failing global column + 1 (the matrix is then
+// not positive definite).  Synchronises s_main at the end.
+int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s_main, hipStream_t s_aux, bool sync = true);
+// z <- L^-1 z (trans=0) or L^-T z (trans=1) with the factor from tile_potrf.
+void tile_trsv(const TileMatrix& L, const CholWorkspace& ws, double* z, int trans, hipStream_t s);
+// sum_i log L_ii (deterministic order).
+double tile_logdet(const TileMatrix& L, double* d_partial, hipStream_t s);
+double device_dot(const double* a, const double* b, long n, double* d_partial, hipStream_t s);
+
+// Everything one MLE_data needs on the device.
+class Context {
+ public:
+  explicit Context(int N, int ts, int nZobs, int nZmiss);
+  ~Context();
+  hipStream_t s1 = nullptr, s2 = nullptr;
+  int N, ts;
+  TileMatrix C;                // Sigma (lower)
+  DeviceVector Z, Zcpy, work;  // observations, pristine copy, solve vector
+  DeviceLocations l1;
+  CholWorkspace ws;
+  double* d_partial = nullptr;  // reduction partial sums
+  long partial_len = 0;
+  // prediction
+  TileMatrix C22, C12;
+  DeviceVector Zobs, Zmiss, Zactual, gemv_part;
+  DeviceLocations lobs, lmiss;
+  PhaseTimes last;
+
+  double loglik(KernelId kid, const double* theta, int metric, double* logdet_out, double* dotp_out, int* info);
+  void ensure_partial(long n);
+};
+
+double wall_seconds();
+
+}  // namespace exa

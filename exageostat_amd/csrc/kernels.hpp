@@ -1,0 +1,69 @@
+// Kernel parameter blocks and launcher declarations.
+#pragma once
+
+#include "besselk.hpp"
+#include "common.hpp"
+
+namespace exa {
+
+// Univariate Matern (optionally with nugget) for one (row set, column set)
+// pair of location arrays; theta-derived constants are prepared on the host.
+struct MaternParams {
+  const double* x1;
+  const double* y1;
+  const double* x2;
+  const double* y2;
+  int metric;     // 0 Euclidean, 1 great circle
+  double beta;    // range
+  double sigma2;  // variance
+  double nugget;  // added where the distance is exactly 0
+  double con;     // 1/(2^(nu-1) Gamma(nu))
+  KnuOrder<double> ko;
+};
+
+struct BivariateParams {
+  const double* x1;
+  const double* y1;
+  const double* x2;
+  const double* y2;
+  int metric;
+  double beta;
+  double sigma11, sigma22, sigma12;  // variances and rho~ sqrt(s1 s2)
+  double con11, con22, con12;
+  KnuOrder<double> ko11, ko22, ko12;
+};
+
+// Generation: ntiles tiles starting at tile column first_col (lower
+// matrices: the trailing triangle from first_col; full matrices: all).
+void launch_gen_matern(const TileView& v, const MaternParams& P, int first_col, long ntiles, hipStream_t s);
+void launch_gen_bivariate(const TileView& v, const BivariateParams& P, int first_col, long ntiles, hipStream_t s);
+
+// ---- tile Cholesky building blocks (FP64) -------------------------------
+// Factor the diagonal tile (k,k) of a lower-packed matrix in place with
+// 64-wide sub-blocks; writes the inverses of the 64x64 diagonal blocks of
+// L_kk to dinv_k (q blocks of 64x64, column-major) and the first failing
+// global column + 1 to *info (left untouched on success).
+void launch_potrf_tile(double* T, int ts, double* dinv_k, int* info, long col_base, hipStream_t s);
+// A(m,k) <- A(m,k) L_kk^-T for all m in (k, mt).
+void launch_trsm_panel(const TileView& v, int k, const double* dinv_k, hipStream_t s);
+// A(m,n) -= A(m,k) A(n,k)^T for n in [col_lo, col_hi], m in [n, mt).
+void launch_tile_update(const TileView& v, int k, int col_lo, int col_hi, hipStream_t s);
+
+// ---- vectors --------------------------------------------------------------
+// z_k <- L_kk^-1 z_k (single tile), then z_m -= L(m,k) z_k for m > k.
+void launch_trsv_tile(const double* T, int ts, const double* dinv_k, double* zk, int trans, hipStream_t s);
+void launch_gemv_panel(const TileView& v, int k, double* z, hipStream_t s);
+// z_m -= L(k,m)^T z_k for m < k.
+void launch_gemv_t_row(const TileView& v, int k, double* z, hipStream_t s);
+// partial[k] = sum_i log(L_kk[i,i]) over positive pivots of each diagonal tile.
+void launch_logdet(const TileView& v, double* partial, hipStream_t s);
+// partial[b] = sum of z[i]*w[i] over 4096-element blocks.
+void launch_dot(const double* z, const double* w, long n, double* partial, hipStream_t s);
+// y[0:m) = A x for a full tile matrix, or for a lower-packed one whose
+// diagonal tiles are lower triangular (zero strictly-upper part); x is
+// padded to nt*ts, part holds nt*mt*ts doubles of scratch.
+void launch_gemv_full(const TileView& v, const double* x, double* y, double* part, hipStream_t s);
+// Zero the strictly upper part of every diagonal tile of a lower matrix.
+void launch_zero_upper_diag(const TileView& v, hipStream_t s);
+
+}  // namespace exa

@@ -1,0 +1,170 @@
+// FP64 MFMA work-group GEMM core for CDNA4 (gfx950).
+//
+// acc += sign * A * B^T, A: BM x K, B: BN x K, both column-major in global
+// memory.  A and B are staged through LDS in BK-deep slices with double
+// buffering (global loads of slice k+1 are in flight while slice k feeds the
+// matrix cores).  Each 64-lane wave owns a WM x WN sub-block made of 16x16
+// fragments of v_mfma_f64_16x16x4f64.
+#pragma once
+
+#include "common.hpp"
+
+namespace exa {
+
+typedef double dx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dx4 mfma_f64(double a, double b, dx4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// v_mfma_f64_16x16x4f64 operand/result layouts (64 lanes):
+//   A (16x4): lane l supplies A[l%16][l/16];  B (4x16): B[l/16][l%16];
+//   D (16x16): lane l, element v holds D[l/16 + 4*v][l%16]
+//   (measured on gfx950 with an in-bounds single-wave probe).
+__device__ __forceinline__ int frag_row(int lane, int v) { return (lane >> 4) + 4 * v; }
+__device__ __forceinline__ int frag_col(int lane) { return lane & 15; }
+
+template <int BM_, int BN_, int WGM_, int WGN_, int BK_>
+struct GemmCfg {
+  static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, BK = BK_;
+  static constexpr int NW = WGM * WGN;
+  static constexpr int NT = NW * 64;
+  static constexpr int WM = BM / WGM, WN = BN / WGN;
+  static constexpr int FM = WM / 16, FN = WN / 16;
+  static constexpr int PAD = 4;
+  static constexpr int LDA = BM + PAD, LDB = BN + PAD;
+  static constexpr int A_STAGE = BK * LDA, B_STAGE = BK * LDB;
+  static constexpr int LDS_DOUBLES = 2 * (A_STAGE + B_STAGE);
+  static constexpr int A_LOADS = BM * BK / 2 / NT;
+  static constexpr int B_LOADS = BN * BK / 2 / NT;
+  static_assert(WM % 16 == 0 && WN % 16 == 0, "wave tile must be 16-aligned");
+  static_assert(A_LOADS * NT * 2 == BM * BK, "A slice must split evenly");
+  static_assert(B_LOADS * NT * 2 == BN * BK, "B slice must split evenly");
+};
+
+template <class C>
+__device__ __forceinline__ void zero_acc(dx4 (&acc)[C::FM][C::FN]) {
+#pragma unroll
+  for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < C::FN; ++j) acc[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
+}
+
+// K must be a multiple of BK; lda/ldb even and A/B 16-byte aligned.
+// Ends with a barrier, so the caller may reuse the LDS afterwards.
+template <class C>
+__device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const double* __restrict__ A, long lda,
+                                            const double* __restrict__ B, long ldb, int K, double* lds,
+                                            double sign) {
+  if (K <= 0) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave / C::WGN;
+  const int wn = wave - wm * C::WGN;
+  double* As = lds;
+  double* Bs = lds + 2 * C::A_STAGE;
+  double2 ra[C::A_LOADS];
+  double2 rb[C::B_LOADS];
+
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int t = 0; t < C::A_LOADS; ++t) {
+      const int e = tid + t * C::NT;
+      const int c = e / (C::BM / 2);
+      const int r2 = e - c * (C::BM / 2);
+      ra[t] = *reinterpret_cast<const double2*>(A + (long)(k0 + c) * lda + 2 * r2);
+    }
+#pragma unroll
+    for (int t = 0; t < C::B_LOADS; ++t) {
+      const int e = tid + t * C::NT;
+      const int c = e / (C::BN / 2);
+      const int r2 = e - c * (C::BN / 2);
+      rb[t] = *reinterpret_cast<const double2*>(B + (long)(k0 + c) * ldb + 2 * r2);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < C::A_LOADS; ++t) {
+      const int e = tid + t * C::NT;
+      const int c = e / (C::BM / 2);
+      const int r2 = e - c * (C::BM / 2);
+      *reinterpret_cast<double2*>(As + buf * C::A_STAGE + c * C::LDA + 2 * r2) = ra[t];
+    }
+#pragma unroll
+    for (int t = 0; t < C::B_LOADS; ++t) {
+      const int e = tid + t * C::NT;
+      const int c = e / (C::BN / 2);
+      const int r2 = e - c * (C::BN / 2);
+      *reinterpret_cast<double2*>(Bs + buf * C::B_STAGE + c * C::LDB + 2 * r2) = rb[t];
+    }
+  };
+
+  const int nk = K / C::BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    const int buf = kc & 1;
+    if (kc + 1 < nk) gload((kc + 1) * C::BK);
+    const double* Ab = As + buf * C::A_STAGE + wm * C::WM + (lane & 15);
+    const double* Bb = Bs + buf * C::B_STAGE + wn * C::WN + (lane & 15);
+#pragma unroll
+    for (int kk = 0; kk < C::BK; kk += 4) {
+      const int krow = kk + (lane >> 4);
+      double a[C::FM], b[C::FN];
+#pragma unroll
+      for (int i = 0; i < C::FM; ++i) a[i] = sign * Ab[krow * C::LDA + i * 16];
+#pragma unroll
+      for (int j = 0; j < C::FN; ++j) b[j] = Bb[krow * C::LDB + j * 16];
+#pragma unroll
+      for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+        for (int j = 0; j < C::FN; ++j) acc[i][j] = mfma_f64(a[i], b[j], acc[i][j]);
+    }
+    if (kc + 1 < nk) sstore(buf ^ 1);
+    __syncthreads();
+  }
+}
+
+// C[r + c*ldc] = beta*C + acc for the block owned by this work-group.
+template <class C>
+__device__ __forceinline__ void acc_update(const dx4 (&acc)[C::FM][C::FN], double* Cp, long ldc, double beta) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave / C::WGN;
+  const int wn = wave - wm * C::WGN;
+#pragma unroll
+  for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < C::FN; ++j) {
+      const int c = wn * C::WN + j * 16 + frag_col(lane);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = wm * C::WM + i * 16 + frag_row(lane, v);
+        double* p = Cp + r + (long)c * ldc;
+        *p = (beta == 0.0 ? 0.0 : beta * *p) + acc[i][j][v];
+      }
+    }
+}
+
+template <class C>
+__device__ __forceinline__ void acc_load(dx4 (&acc)[C::FM][C::FN], const double* Cp, long ldc) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave / C::WGN;
+  const int wn = wave - wm * C::WGN;
+#pragma unroll
+  for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < C::FN; ++j) {
+      const int c = wn * C::WN + j * 16 + frag_col(lane);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = wm * C::WM + i * 16 + frag_row(lane, v);
+        acc[i][j][v] = Cp[r + (long)c * ldc];
+      }
+    }
+}
+
+}  // namespace exa

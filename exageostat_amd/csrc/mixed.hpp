@@ -1,0 +1,20 @@
+// Mixed-precision likelihood: FP64 diagonal tiles, FP32/BF16 off-diagonal
+// tiles chosen per theta, FP64 iterative refinement of the solve.
+#pragma once
+
+#include "engine.hpp"
+
+namespace exa {
+
+struct MixedOptions {
+  double fp64_corr = 0.0;  // off-diagonal tiles with max correlation >= this stay FP64 (0: none)
+  double fp32_corr = 0.0;  // ... >= this are FP32, the rest BF16 (when bf16)
+  bool bf16 = false;
+  int ir_steps = 3;
+  int diag_thick = 1;
+};
+
+double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric, const MixedOptions& opt,
+                    double* logdet_out, double* dotp_out, int* info);
+
+}  // namespace exa

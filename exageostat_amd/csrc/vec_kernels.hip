@@ -1,0 +1,261 @@
+// Vector-side kernels of the likelihood: blocked triangular solves with the
+// tile factor, log-determinant and dot-product partial sums, and the
+// rectangular matrix-vector product of kriging.
+//
+// Replaces, from EXAGEOSTAT_dmle_Tile (exageostat_exact/src/compute/
+// MLE_dexact.c:381-399): the dmdet reduction (codelet_zmdet.c:108-139,
+// core_zmdet.c:50-59), CHAMELEON_dtrsm_Tile(Left, Lower, NoTrans, NonUnit)
+// on Z and the 1x1 CHAMELEON_dgemm_Tile dot product; from
+// EXAGEOSTAT_dmle_Predict_Tile (:812-820): the two triangular solves of
+// dposv and the dgemm C12 * alpha.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace exa {
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Single tile: z <- L^-1 z (trans = 0) or z <- L^-T z (trans = 1), L lower
+// ts x ts (column-major, ld = ts) with inverses of its 64x64 diagonal blocks
+// in dinv.  One work-group; z lives in LDS during the sweep.
+__global__ void __launch_bounds__(256) k_trsv_tile(const double* T, int ts, const double* dinv, double* z,
+                                                   int trans) {
+  __shared__ double y[1024];
+  __shared__ double part[4][64];
+  __shared__ double t[64];
+  const int tid = threadIdx.x;
+  const int r = tid & 63;
+  const int g = tid >> 6;
+  const int q = ts / kNB;
+  for (int i = tid; i < ts; i += 256) y[i] = z[i];
+  __syncthreads();
+  if (!trans) {
+    for (int j = 0; j < q; ++j) {
+      const int row = j * kNB + r;
+      double s = 0.0;
+      for (int c = g; c < j * kNB; c += 4) s += T[row + (long)c * ts] * y[c];
+      part[g][r] = s;
+      __syncthreads();
+      if (tid < 64) t[r] = y[row] - ((part[0][r] + part[1][r]) + (part[2][r] + part[3][r]));
+      __syncthreads();
+      if (tid < 64) {
+        const double* inv = dinv + (long)j * 4096;
+        double acc = 0.0;
+        for (int c = 0; c <= r; ++c) acc += inv[r + c * 64] * t[c];
+        y[row] = acc;
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int j = q - 1; j >= 0; --j) {
+      // s[col] = sum_{rows below block j} L[row][col] y[row] for the 64
+      // columns col of block j; each wave owns 16 columns, lanes run rows.
+      for (int cc = g * 16; cc < g * 16 + 16; ++cc) {
+        const int col = j * kNB + cc;
+        double s = 0.0;
+        for (int row = (j + 1) * kNB + r; row < ts; row += 64) s += T[row + (long)col * ts] * y[row];
+        s = wave_sum(s);
+        if (r == 0) part[0][cc] = s;
+      }
+      __syncthreads();
+      if (tid < 64) t[r] = y[j * kNB + r] - part[0][r];
+      __syncthreads();
+      if (tid < 64) {
+        // y_j = inv^T t: y[r] = sum_c inv[c][r] t[c] = sum_{c >= r} dinv[c + r*64] t[c]
+        const double* inv = dinv + (long)j * 4096;
+        double acc = 0.0;
+        for (int c = r; c < 64; ++c) acc += inv[c + r * 64] * t[c];
+        y[j * kNB + r] = acc;
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < ts; i += 256) z[i] = y[i];
+}
+
+// z_m -= L(m,k) z_k for m in (k, mt); one work-group per 64-row block.
+__global__ void __launch_bounds__(256) k_gemv_panel(TileView v, int k, double* z) {
+  __shared__ double y[1024];
+  __shared__ double part[4][64];
+  const int tid = threadIdx.x;
+  const int r = tid & 63;
+  const int g = tid >> 6;
+  const int ts = v.ts;
+  const int q = ts / kNB;
+  const int m = k + 1 + blockIdx.x / q;
+  const int rb = blockIdx.x - (blockIdx.x / q) * q;
+  const double* zk = z + (long)k * ts;
+  for (int i = tid; i < ts; i += 256) y[i] = zk[i];
+  __syncthreads();
+  const double* A = tile_f64(v, m, k) + rb * kNB + r;
+  double s0 = 0.0, s1 = 0.0;
+  int c = g;
+  for (; c + 4 < ts; c += 8) {
+    s0 += A[(long)c * ts] * y[c];
+    s1 += A[(long)(c + 4) * ts] * y[c + 4];
+  }
+  for (; c < ts; c += 4) s0 += A[(long)c * ts] * y[c];
+  part[g][r] = s0 + s1;
+  __syncthreads();
+  if (tid < 64) {
+    z[(long)m * ts + rb * kNB + r] -= (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
+  }
+}
+
+// z_m -= L(k,m)^T z_k for m in [0, k); one work-group per 64 columns of L(k,m).
+__global__ void __launch_bounds__(256) k_gemv_t_row(TileView v, int k, double* z) {
+  __shared__ double y[1024];
+  __shared__ double part[64];
+  const int tid = threadIdx.x;
+  const int r = tid & 63;
+  const int g = tid >> 6;
+  const int ts = v.ts;
+  const int q = ts / kNB;
+  const int m = blockIdx.x / q;
+  const int cb = blockIdx.x - m * q;
+  const double* zk = z + (long)k * ts;
+  for (int i = tid; i < ts; i += 256) y[i] = zk[i];
+  __syncthreads();
+  const double* A = tile_f64(v, k, m);
+  for (int cc = g * 16; cc < g * 16 + 16; ++cc) {
+    const int col = cb * kNB + cc;
+    double s = 0.0;
+    for (int row = r; row < ts; row += 64) s += A[row + (long)col * ts] * y[row];
+    s = wave_sum(s);
+    if (r == 0) part[cc] = s;
+  }
+  __syncthreads();
+  if (tid < 64) z[(long)m * ts + cb * kNB + r] -= part[r];
+}
+
+__global__ void __launch_bounds__(256) k_logdet(TileView v, double* partial) {
+  __shared__ double red[256];
+  const int k = blockIdx.x;
+  const double* T = tile_f64(v, k, k);
+  double s = 0.0;
+  for (int i = threadIdx.x; i < v.ts; i += 256) {
+    const long gi = (long)k * v.ts + i;
+    const double d = T[i + (long)i * v.ts];
+    if (gi < v.m && d > 0.0) s += log(d);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[k] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_dot(const double* z, const double* w, long n, double* partial) {
+  __shared__ double red[256];
+  const long base = (long)blockIdx.x * 4096;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < 4096; i += 256) {
+    const long gi = base + i;
+    if (gi < n) s += z[gi] * w[gi];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int ww = 128; ww > 0; ww >>= 1) {
+    if (threadIdx.x < ww) red[threadIdx.x] += red[threadIdx.x + ww];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// y_partial[n][row] = sum over tile column n of A(m,n)[row] x_n; one
+// work-group per (tile row, 64-row block, tile column).
+__global__ void __launch_bounds__(256) k_gemv_full_partial(TileView v, const double* x, double* part_out) {
+  __shared__ double part[4][64];
+  const int tid = threadIdx.x;
+  const int r = tid & 63;
+  const int g = tid >> 6;
+  const int ts = v.ts;
+  const int q = ts / kNB;
+  const int n = blockIdx.y;
+  const int m = blockIdx.x / q;
+  const int rb = blockIdx.x - m * q;
+  double s = 0.0;
+  if (!(v.lower && n > m)) {
+    const double* A = tile_f64(v, m, n) + rb * kNB + r;
+    const double* xn = x + (long)n * ts;
+    for (int c = g; c < ts; c += 4) s += A[(long)c * ts] * xn[c];
+  }
+  part[g][r] = s;
+  __syncthreads();
+  if (tid < 64) {
+    part_out[(long)n * ((long)v.mt * ts) + (long)m * ts + rb * kNB + r] =
+        (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
+  }
+}
+
+__global__ void k_sum_columns(const double* part, int ncols, long len, double* y, long m) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int c = 0; c < ncols; ++c) s += part[(long)c * len + i];
+  y[i] = s;
+}
+
+__global__ void __launch_bounds__(256) k_zero_upper_diag(TileView v) {
+  const int k = blockIdx.y;
+  double* T = tile_f64(v, k, k);
+  const int c = blockIdx.x;
+  for (int r = threadIdx.x; r < c; r += 256) T[r + (long)c * v.ts] = 0.0;
+}
+
+void launch_zero_upper_diag(const TileView& v, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_upper_diag, dim3(v.ts, v.mt), dim3(256), 0, s, v);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_trsv_tile(const double* T, int ts, const double* dinv_k, double* zk, int trans, hipStream_t s) {
+  hipLaunchKernelGGL(k_trsv_tile, dim3(1), dim3(256), 0, s, T, ts, dinv_k, zk, trans);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_gemv_panel(const TileView& v, int k, double* z, hipStream_t s) {
+  const int q = v.ts / kNB;
+  const long blocks = (long)(v.mt - k - 1) * q;
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(k_gemv_panel, dim3((unsigned)blocks), dim3(256), 0, s, v, k, z);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_gemv_t_row(const TileView& v, int k, double* z, hipStream_t s) {
+  const int q = v.ts / kNB;
+  const long blocks = (long)k * q;
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(k_gemv_t_row, dim3((unsigned)blocks), dim3(256), 0, s, v, k, z);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_logdet(const TileView& v, double* partial, hipStream_t s) {
+  hipLaunchKernelGGL(k_logdet, dim3(v.mt), dim3(256), 0, s, v, partial);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_dot(const double* z, const double* w, long n, double* partial, hipStream_t s) {
+  const long blocks = (n + 4095) / 4096;
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(k_dot, dim3((unsigned)blocks), dim3(256), 0, s, z, w, n, partial);
+  EXA_HIP(hipGetLastError());
+}
+
+// part must hold nt * mt * ts doubles.
+void launch_gemv_full(const TileView& v, const double* x, double* y, double* part, hipStream_t s) {
+  const int q = v.ts / kNB;
+  hipLaunchKernelGGL(k_gemv_full_partial, dim3((unsigned)(v.mt * q), (unsigned)v.nt), dim3(256), 0, s, v, x,
+                     part);
+  const long len = (long)v.mt * v.ts;
+  hipLaunchKernelGGL(k_sum_columns, dim3((unsigned)((v.m + 255) / 256)), dim3(256), 0, s, part, v.nt, len, y,
+                     (long)v.m);
+  EXA_HIP(hipGetLastError());
+}
+
+}  // namespace exa

@@ -1,0 +1,253 @@
+"""Python mirror of the reference's driver-level interface for the exact
+likelihood path (examples/synthetic_dmle_test.c, src/compute/MLE.c).
+
+Every method calls the C ABI of libexageostat_gpu.so; nothing here computes
+covariances, factorisations or likelihoods itself.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import MLEData, Desc, c_double_p
+
+KERNEL_PARAMS = {
+    "univariate_matern_stationary": 3,
+    "univariate_matern_nuggets_stationary": 4,
+    "bivariate_matern_parsimonious": 6,
+}
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(c_double_p)
+
+
+def _check(lib, what: str):
+    st = lib.exageostat_gpu_last_status()
+    if st < 0:
+        raise RuntimeError(f"{what} failed ({st}): {lib.exageostat_gpu_last_error().decode()}")
+    return st
+
+
+def device_count() -> int:
+    return _lib.load().exageostat_gpu_device_count()
+
+
+def generate_xy_loc(n: int, seed: int):
+    """GenerateXYLoc through the library (glibc srand/rand + Morton sort)."""
+    lib = _lib.load()
+    loc = lib.GenerateXYLoc(n, seed)
+    x = np.ctypeslib.as_array(loc.contents.x, shape=(n,)).copy()
+    y = np.ctypeslib.as_array(loc.contents.y, shape=(n,)).copy()
+    libc = ctypes.CDLL("libc.so.6")
+    libc.free.argtypes = [ctypes.c_void_p]
+    libc.free(ctypes.cast(loc.contents.x, ctypes.c_void_p))
+    libc.free(ctypes.cast(loc.contents.y, ctypes.c_void_p))
+    libc.free(ctypes.cast(loc, ctypes.c_void_p))
+    return x, y
+
+
+class ExactMLE:
+    """One MLE_data with its GPU context: EXAGEOSTAT_dmle_Call at
+    construction, MLE_Finalize at close()."""
+
+    def __init__(self, x, y, ts: int, kernel: str = "univariate_matern_stationary", dm: str = "ed",
+                 precision: int = 0, silent: bool = True):
+        self.lib = _lib.load()
+        if kernel not in KERNEL_PARAMS:
+            raise ValueError(f"unsupported kernel {kernel!r}")
+        self.kernel = kernel
+        self.nparams = KERNEL_PARAMS[kernel]
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.nloc = len(self.x)
+        self.N = self.nloc * (2 if kernel.startswith("bivariate") else 1)
+        self.ts = ts
+        self._strings = [b"exact", kernel.encode(), dm.encode(), b""]
+        d = MLEData()
+        d.computation = self._strings[0]
+        d.kernel_fun = self._strings[1]
+        d.dm = self._strings[2]
+        d.recovery_file = self._strings[3]
+        d.precision = precision
+        d.silent = 1 if silent else 0
+        d.l1.x = _dp(self.x)
+        d.l1.y = _dp(self.y)
+        d.l1.z = None
+        d.diag_thick = 1
+        d.mp_ir_steps = 3
+        self.data = d
+        zero = ctypes.c_int(0)
+        one = ctypes.c_int(1)
+        tsi = ctypes.c_int(ts)
+        self.lib.exageostat_init(ctypes.byref(one), ctypes.byref(one), ctypes.byref(tsi), ctypes.byref(zero))
+        _check(self.lib, "exageostat_init")
+        self.lib.EXAGEOSTAT_dmle_Call(ctypes.byref(d), 1, 1, ts, 1, 1, self.N, self.N, 0)
+        _check(self.lib, "EXAGEOSTAT_dmle_Call")
+        self._pred_refs = []
+
+    # -- data ------------------------------------------------------------
+    def set_z(self, z) -> None:
+        """Observations (the real-data path: Lapack_to_Tile into descZ)."""
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        assert z.shape == (self.N,)
+        desc = ctypes.cast(self.data.descZ, ctypes.POINTER(Desc))
+        self.lib.EXAGEOSTAT_Lapack_to_Tile(_dp(z), self.N, desc)
+        _check(self.lib, "Lapack_to_Tile")
+        self.data.iter_count = 0
+
+    def zvg(self, theta, e) -> None:
+        """MLE_zvg: z = L(theta) e on the GPU."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        self.lib.MLE_zvg(ctypes.byref(self.data), _dp(e), _dp(th), self.N, self.ts, 0, 1, 1)
+        _check(self.lib, "MLE_zvg")
+
+    def get_z(self) -> np.ndarray:
+        z = np.empty(self.N)
+        self.lib.MLE_get_zobs(ctypes.byref(self.data), _dp(z), self.N)
+        _check(self.lib, "MLE_get_zobs")
+        return z
+
+    # -- likelihood ------------------------------------------------------
+    def loglik(self, theta, grad: bool = False):
+        """MLE_alg; returns l (and d l/d theta when grad=True)."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        g = np.zeros(len(th)) if grad else None
+        val = self.lib.MLE_alg(len(th), _dp(th), _dp(g) if grad else None, ctypes.byref(self.data))
+        _check(self.lib, "MLE_alg")
+        return (val, g) if grad else val
+
+    @property
+    def status(self) -> int:
+        return self.data.status
+
+    @property
+    def logdet(self) -> float:
+        return self.data.logdet
+
+    @property
+    def dotp(self) -> float:
+        return self.data.dotp
+
+    def phase_times(self) -> dict:
+        d = self.data
+        return {"gen": d.time_gen, "facto": d.time_facto, "logdet": d.time_logdet, "solve": d.time_solve,
+                "total": d.time_total}
+
+    # -- prediction ------------------------------------------------------
+    def predict(self, theta, xobs, yobs, zobs, xmiss, ymiss, zactual=None):
+        """EXAGEOSTAT_dmle_Predict_Tile; returns (zmiss, mse)."""
+        xo = np.ascontiguousarray(xobs, dtype=np.float64)
+        yo = np.ascontiguousarray(yobs, dtype=np.float64)
+        xm = np.ascontiguousarray(xmiss, dtype=np.float64)
+        ym = np.ascontiguousarray(ymiss, dtype=np.float64)
+        zo = np.ascontiguousarray(zobs, dtype=np.float64)
+        rows = 2 if self.kernel.startswith("bivariate") else 1
+        n_miss = len(xm) * rows
+        n_obs = len(xo) * rows
+        zm = np.zeros(n_miss)
+        za = None if zactual is None else np.ascontiguousarray(zactual, dtype=np.float64)
+        self._pred_refs = [xo, yo, xm, ym]
+        d = self.data
+        d.lobs.x, d.lobs.y, d.lobs.z = _dp(xo), _dp(yo), None
+        d.lmiss.x, d.lmiss.y, d.lmiss.z = _dp(xm), _dp(ym), None
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        self.lib.EXAGEOSTAT_dmle_Predict_Allocate(ctypes.byref(d), len(xm), len(xo), self.ts, 1, 1, 1)
+        _check(self.lib, "Predict_Allocate")
+        mse = self.lib.EXAGEOSTAT_dmle_Predict_Tile(ctypes.byref(d), _dp(th), n_miss, n_obs, _dp(zo),
+                                                    _dp(za) if za is not None else None, _dp(zm), self.N)
+        _check(self.lib, "Predict_Tile")
+        return zm, mse
+
+    def close(self) -> None:
+        if self.data is not None and self.data.gpu:
+            self.lib.prediction_finalize(ctypes.byref(self.data))
+            self.lib.MLE_Finalize(ctypes.byref(self.data))
+        self.data = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceMatrix:
+    """A descriptor-level handle (EXAGEOSTAT_Desc_Create) for stage tests:
+    dcmg -> dpotrf -> dmdet -> dtrsv, the reference's call sequence."""
+
+    def __init__(self, m: int, n: int, ts: int, lower: bool = True):
+        self.lib = _lib.load()
+        self.desc = ctypes.POINTER(Desc)()
+        st = self.lib.EXAGEOSTAT_Desc_Create(ctypes.byref(self.desc), _lib.ChamLower if lower else
+                                             _lib.ChamUpperLower, ts, m, n)
+        if st != 0:
+            _check(self.lib, "Desc_Create")
+            raise RuntimeError("Desc_Create failed")
+        self.m, self.n, self.ts, self.lower = m, n, ts, lower
+        self._keep = []
+
+    def dcmg(self, theta, x1, y1, x2=None, y2=None, kernel="univariate_matern_stationary", dm="ed"):
+        x1 = np.ascontiguousarray(x1, dtype=np.float64)
+        y1 = np.ascontiguousarray(y1, dtype=np.float64)
+        x2 = x1 if x2 is None else np.ascontiguousarray(x2, dtype=np.float64)
+        y2 = y1 if y2 is None else np.ascontiguousarray(y2, dtype=np.float64)
+        l1 = _lib.Location(_dp(x1), _dp(y1), None)
+        l2 = _lib.Location(_dp(x2), _dp(y2), None)
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        self._keep = [x1, y1, x2, y2, th]
+        st = self.lib.EXAGEOSTAT_MLE_dcmg_Tile_Async(_lib.ChamLower, self.desc, ctypes.byref(l1), ctypes.byref(l2),
+                                                     None, _dp(th), dm.encode(), kernel.encode(), None, None)
+        _check(self.lib, "dcmg")
+        return st
+
+    def potrf(self) -> int:
+        info = self.lib.EXAGEOSTAT_dpotrf_Tile(_lib.ChamLower, self.desc)
+        _check(self.lib, "dpotrf")
+        return info
+
+    def logdet_half(self) -> float:
+        acc = np.zeros(1)
+        ud = ctypes.POINTER(Desc)()
+        self.lib.EXAGEOSTAT_Desc_Create_User(ctypes.byref(ud), _dp(acc), 1, 1)
+        self.lib.EXAGEOSTAT_MLE_dmdet_Tile_Async(self.desc, None, None, ud)
+        _check(self.lib, "dmdet")
+        self.lib.EXAGEOSTAT_Desc_Destroy(ctypes.byref(ud))
+        return float(acc[0])
+
+    def trsv(self, b, trans: bool = False) -> np.ndarray:
+        b = np.array(b, dtype=np.float64, copy=True)
+        self.lib.EXAGEOSTAT_dtrsv_Tile(_lib.ChamTrans if trans else _lib.ChamNoTrans, self.desc, _dp(b))
+        _check(self.lib, "dtrsv")
+        return b
+
+    def to_numpy(self) -> np.ndarray:
+        A = np.zeros((self.m, self.n), order="F")
+        self.lib.EXAGEOSTAT_Tile_to_Lapack(self.desc, A.ctypes.data_as(c_double_p), self.m)
+        _check(self.lib, "Tile_to_Lapack")
+        return A
+
+    def from_numpy(self, A) -> None:
+        A = np.asfortranarray(A, dtype=np.float64)
+        self.lib.EXAGEOSTAT_Lapack_to_Tile(A.ctypes.data_as(c_double_p), self.m, self.desc)
+        _check(self.lib, "Lapack_to_Tile")
+
+    def close(self):
+        if self.desc:
+            self.lib.EXAGEOSTAT_Desc_Destroy(ctypes.byref(self.desc))
+            self.desc = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def loglik_formula(dotp: float, logdet: float, n: int) -> float:
+    """l = -dotp/2 - logdet/2 - (n/2) log 2 pi (MLE_dexact.c:426)."""
+    return -0.5 * dotp - 0.5 * logdet - (n / 2.0) * math.log(2.0 * math.pi)

@@ -1,0 +1,111 @@
+"""Dense FP64 restatement of the likelihood, prediction and gradient path.
+
+Test infrastructure only (see ``oracle/__init__.py``).
+
+Pinning (tests/test_oracle_likelihood.py):
+* ``dmle`` against an independent multivariate-normal log-density
+  (``scipy.stats.multivariate_normal.logpdf``) and, for n <= 48, against
+  a 40-digit mpmath Cholesky; invariance under a joint permutation of the
+  locations and z.
+* ``predict`` reproduces the observed value at an observed location
+  (noise-free kriging interpolates) and matches an explicit
+  C12 @ inv(C22) @ z.
+* ``gradient`` against central finite differences of ``dmle``.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import scipy.linalg as sla
+
+from . import matern
+
+LOG_2PI = math.log(2.0 * math.pi)
+
+
+def sigma(kernel: str, x, y, theta, metric: int = 0) -> np.ndarray:
+    """Full symmetric covariance of one location set."""
+    return matern.cov_matrix(kernel, x, y, x, y, theta, metric)
+
+
+def dzvg(kernel: str, x, y, theta, e, metric: int = 0) -> np.ndarray:
+    """EXAGEOSTAT_MLE_dzvg_Tile (MLE_dexact.c:25-148): z = L(theta) e with
+    L the lower Cholesky factor of Sigma(theta) (dcmg -> dpotrf -> dtrmm)."""
+    L = np.linalg.cholesky(sigma(kernel, x, y, theta, metric))
+    return L @ np.asarray(e, dtype=np.float64)
+
+
+def loglik_from_sigma(S: np.ndarray, z: np.ndarray):
+    """The steps of EXAGEOSTAT_dmle_Tile after generation
+    (MLE_dexact.c:372-426): dpotrf, logdet = 2 sum log L_ii over positive
+    pivots (core_zmdet.c:50-59), v = L^-1 z (dtrsm), dotp = v.v (dgemm),
+    l = -dotp/2 - logdet/2 - (N/2) log 2 pi.  Returns (l, logdet, dotp)."""
+    L = np.linalg.cholesky(S)
+    d = np.diag(L)
+    logdet = 2.0 * float(np.sum(np.log(d[d > 0])))
+    v = sla.solve_triangular(L, np.asarray(z, dtype=np.float64), lower=True)
+    dotp = float(v @ v)
+    n = S.shape[0]
+    return -0.5 * dotp - 0.5 * logdet - (n / 2.0) * LOG_2PI, logdet, dotp
+
+
+def dmle(kernel: str, x, y, z, theta, metric: int = 0):
+    """EXAGEOSTAT_dmle_Tile (MLE_dexact.c:231-489), dense.
+    Returns (l, logdet, dotp)."""
+    return loglik_from_sigma(sigma(kernel, x, y, theta, metric), z)
+
+
+def predict(kernel: str, xobs, yobs, zobs, xmiss, ymiss, theta, zactual=None, metric: int = 0):
+    """EXAGEOSTAT_dmle_Predict_Tile (MLE_dexact.c:688-874):
+    C22 = Sigma(lobs, lobs), C12 = Sigma(lmiss, lobs), zmiss = C12 C22^-1 zobs
+    (dposv + dgemm); mse = sum (zactual - zmiss)^2 / nZmiss
+    (codelet_zmse.c:22-38, MLE_dexact.c:841-847).  For the bivariate kernel
+    nZmiss counts rows (2 per location) and the per-variable errors are
+    divided by nZmiss/2 (codelet_zmse_bivariate.c:22-51).
+    Returns (zmiss, mse, mse1, mse2)."""
+    C22 = matern.cov_matrix(kernel, xobs, yobs, xobs, yobs, theta, metric)
+    C12 = matern.cov_matrix(kernel, xmiss, ymiss, xobs, yobs, theta, metric)
+    cf = sla.cho_factor(C22, lower=True)
+    alpha = sla.cho_solve(cf, np.asarray(zobs, dtype=np.float64))
+    zmiss = C12 @ alpha
+    if zactual is None:
+        return zmiss, -1.0, None, None
+    err = (np.asarray(zactual, dtype=np.float64) - zmiss) ** 2
+    nm = len(zmiss)
+    mse = float(np.sum(err)) / nm
+    if kernel.startswith("bivariate"):
+        return zmiss, mse, float(np.sum(err[0::2])) / (nm // 2), float(np.sum(err[1::2])) / (nm // 2)
+    return zmiss, mse, None, None
+
+
+def gradient(kernel: str, x, y, z, theta, metric: int = 0):
+    """d l / d theta = 1/2 alpha^T dS alpha - 1/2 tr(S^-1 dS), alpha = S^-1 z,
+    for the univariate Matern kernels (3 parameters, or 4 with the nugget).
+    Returns (l, grad)."""
+    nugget = kernel == "univariate_matern_nuggets_stationary"
+    S = sigma(kernel, x, y, theta, metric)
+    cf = sla.cho_factor(S, lower=True)
+    alpha = sla.cho_solve(cf, np.asarray(z, dtype=np.float64))
+    Sinv = sla.cho_solve(cf, np.eye(S.shape[0]))
+    dS = matern.matern_derivatives(x, y, x, y, theta, metric, nugget=nugget)
+    g = np.array([0.5 * alpha @ D @ alpha - 0.5 * float(np.sum(Sinv * D)) for D in dS])
+    l, _, _ = loglik_from_sigma(S, z)
+    return l, g
+
+
+def flops_dpotrf(n: float) -> float:
+    """FLOPS_DPOTRF (include/flops.h:97-98,254)."""
+    n = float(n)
+    fmuls = n * (((1.0 / 6.0) * n + 0.5) * n + (1.0 / 3.0))
+    fadds = n * (((1.0 / 6.0) * n) * n - (1.0 / 6.0))
+    return fmuls + fadds
+
+
+def flops_dtrsm_left(m: float, nrhs: float) -> float:
+    """FLOPS_DTRSM(ChamLeft, m, nrhs) (include/flops.h:73-81,231).  The
+    header defines FADDS_TRSM as FMULS_TRMM, so both halves count
+    nrhs*m(m+1)/2 and the total is nrhs*m*(m+1)."""
+    m = float(m)
+    nrhs = float(nrhs)
+    return nrhs * m * (m + 1.0)

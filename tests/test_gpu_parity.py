@@ -1,0 +1,226 @@
+"""GPU parity of the FP64 path against the CPU oracle and the golden fixtures.
+
+Every test calls the HIP library through its C ABI (ctypes); the oracle is
+only the checker.  Tolerances are stated per test.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import datagen, likelihood, matern
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+K = "univariate_matern_stationary"
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import exageostat_amd
+    exageostat_amd.load()
+    assert exageostat_amd.device_count() >= 1, "no HIP device visible"
+    return exageostat_amd
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return dict(np.load(os.path.join(GOLD, "c1_seed0.npz")))
+
+
+def _locs(n, seed=0):
+    x, y, _ = datagen.generate_xy_loc(n, seed)
+    return x, y
+
+
+# ---------------------------------------------------------------- generation
+@pytest.mark.parametrize("theta", [(1.0, 0.1, 0.5), (1.5, 0.03, 1.0), (0.7, 0.2, 1.37), (1.0, 0.1, 2.5),
+                                   (2.0, 0.05, 3.3), (1.0, 0.3, 0.27)])
+def test_dcmg_matches_oracle(lib, theta):
+    """Lower tiles of Sigma: entries within 1e-13 * sigma^2 of the oracle
+    (scipy kv, itself pinned to mpmath), ts=64 with a partial last tile."""
+    n, ts = 700, 64
+    x, y = _locs(n, 3)
+    M = lib.DeviceMatrix(n, n, ts, lower=True)
+    M.dcmg(theta, x, y)
+    A = M.to_numpy()
+    S = matern.core_dcmg(x, y, x, y, theta)
+    mask = np.zeros((n, n), dtype=bool)
+    for tn in range(M.desc.contents.nt):
+        mask[tn * ts:, tn * ts:(tn + 1) * ts] = True
+    err = np.max(np.abs(A[mask] - S[mask]))
+    assert err <= 1e-13 * theta[0], err
+
+
+def test_dcmg_rectangular_and_nugget(lib):
+    """Full (non-square) descriptor, nugget kernel: C12 between two sets."""
+    x1, y1 = _locs(130, 1)
+    x2, y2 = _locs(300, 2)
+    theta = (1.2, 0.15, 0.9, 0.05)
+    M = lib.DeviceMatrix(130, 300, 64, lower=False)
+    M.dcmg(theta, x1, y1, x2, y2, kernel="univariate_matern_nuggets_stationary")
+    A = M.to_numpy()
+    S = matern.core_dcmg_nuggets(x1, y1, x2, y2, theta)
+    assert np.max(np.abs(A - S)) <= 1e-13 * 1.25
+
+
+def test_dcmg_great_circle(lib):
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-80, 80, 200)
+    y = rng.uniform(-170, 170, 200)
+    theta = (1.0, 500.0, 0.8)
+    M = lib.DeviceMatrix(200, 200, 64, lower=False)
+    M.dcmg(theta, x, y, dm="gc")
+    S = matern.core_dcmg(x, y, x, y, theta, metric=1)
+    assert np.max(np.abs(M.to_numpy() - S)) <= 1e-12
+
+
+def test_dcmg_bivariate(lib):
+    x, y = _locs(200, 4)
+    theta = (1.0, 1.5, 0.1, 0.5, 1.0, 0.5)
+    M = lib.DeviceMatrix(400, 400, 64, lower=False)
+    M.dcmg(theta, x, y, kernel="bivariate_matern_parsimonious")
+    S = matern.core_dcmg_bivariate_parsimonious(x, y, x, y, theta)
+    assert np.max(np.abs(M.to_numpy() - S)) <= 1e-13 * 1.5
+
+
+# ---------------------------------------------------------------- Cholesky
+@pytest.mark.parametrize("n,ts", [(700, 64), (1600, 320), (2500, 960), (2300, 1024), (1000, 192)])
+def test_potrf_matches_lapack(lib, n, ts):
+    """Tile Cholesky vs LAPACK: ||L - L_lapack||_F / ||L||_F <= 1e-12; both
+    trailing-update tilings (192- and 128-wide blocks) and the 64-wide one."""
+    x, y = _locs(n, 0)
+    theta = (1.0, 0.1, 1.0)
+    S = matern.core_dcmg(x, y, x, y, theta)
+    M = lib.DeviceMatrix(n, n, ts, lower=True)
+    M.from_numpy(S)
+    info = M.potrf()
+    assert info == 0
+    L = np.tril(M.to_numpy())
+    Lr = np.linalg.cholesky(S)
+    rel = np.linalg.norm(L - Lr) / np.linalg.norm(Lr)
+    assert rel <= 1e-12, rel
+    # logdet through dmdet and the forward/backward solves
+    ld = 2.0 * M.logdet_half()
+    assert abs(ld - 2.0 * np.sum(np.log(np.diag(Lr)))) <= 1e-10 * abs(ld)
+    b = np.sin(np.arange(n) * 0.37)
+    v = M.trsv(b)
+    vr = sla.solve_triangular(Lr, b, lower=True)
+    assert np.linalg.norm(v - vr) <= 1e-10 * np.linalg.norm(vr)
+    w = M.trsv(vr, trans=True)
+    wr = sla.solve_triangular(Lr, vr, lower=True, trans="T")
+    assert np.linalg.norm(w - wr) <= 1e-10 * np.linalg.norm(wr)
+
+
+def test_potrf_reports_non_spd(lib):
+    n, ts = 300, 64
+    x, y = _locs(n, 0)
+    S = matern.core_dcmg(x, y, x, y, (1.0, 0.1, 0.5))
+    S[130, 130] = -1.0
+    M = lib.DeviceMatrix(n, n, ts, lower=True)
+    M.from_numpy(S)
+    info = M.potrf()
+    # first failing column (1-based) must be 131 at the latest
+    assert 0 < info <= 131
+
+
+# ---------------------------------------------------------------- likelihood
+def test_loglik_c1_golden(lib, c1):
+    """C1 (n=1600, ts=320): |dl|/|l| <= 1e-9 against the FP64 oracle for the
+    thetas whose Sigma has cond <= 2e6.  The last golden theta (0.8, 0.2,
+    2.5) has cond(Sigma) = 2.2e10: perturbing Sigma's entries by one ulp
+    already moves l by 1.8e-9 relative in the oracle itself, so it gets
+    1e-7."""
+    m = lib.ExactMLE(c1["x"], c1["y"], 320)
+    m.set_z(c1["z"])
+    tols = [1e-9, 1e-9, 1e-9, 1e-9, 1e-7]
+    for th, ll, ld, dp, tol in zip(c1["thetas"], c1["loglik"], c1["logdet"], c1["dotp"], tols):
+        val = m.loglik(th)
+        assert m.status == 0
+        assert abs(val - ll) <= tol * abs(ll), (th, val, ll)
+        assert abs(m.logdet - ld) <= tol * max(1.0, abs(ld))
+        assert abs(m.dotp - dp) <= tol * abs(dp)
+    m.close()
+
+
+@pytest.mark.parametrize("ts", [64, 320, 960])
+def test_loglik_tile_size_invariance(lib, c1, ts):
+    m = lib.ExactMLE(c1["x"], c1["y"], ts)
+    m.set_z(c1["z"])
+    val = m.loglik(c1["theta_true"])
+    assert abs(val - c1["loglik"][0]) <= 1e-9 * abs(c1["loglik"][0])
+    m.close()
+
+
+def test_zvg_c1_golden(lib, c1):
+    """MLE_zvg: z = L e on the GPU equals the oracle's z (rel 1e-12)."""
+    m = lib.ExactMLE(c1["x"], c1["y"], 320)
+    m.zvg(c1["theta_true"], c1["e"])
+    z = m.get_z()
+    assert np.linalg.norm(z - c1["z"]) <= 1e-12 * np.linalg.norm(c1["z"])
+    val = m.loglik(c1["theta_true"])
+    assert abs(val - c1["loglik"][0]) <= 1e-9 * abs(c1["loglik"][0])
+    m.close()
+
+
+def test_loglik_permutation_invariance(lib, c1):
+    """l is invariant under a joint permutation of locations and z."""
+    p = np.random.default_rng(7).permutation(1600)
+    m = lib.ExactMLE(c1["x"][p], c1["y"][p], 320)
+    m.set_z(c1["z"][p])
+    val = m.loglik(c1["theta_true"])
+    assert abs(val - c1["loglik"][0]) <= 1e-9 * abs(c1["loglik"][0])
+    m.close()
+
+
+def test_loglik_non_spd_status(lib, c1):
+    m = lib.ExactMLE(c1["x"], c1["y"], 320)
+    m.set_z(c1["z"])
+    val = m.loglik([-1.0, 0.1, 0.5])
+    assert val == -np.inf
+    assert m.status > 0
+    # the context stays usable
+    val = m.loglik(c1["theta_true"])
+    assert abs(val - c1["loglik"][0]) <= 1e-9 * abs(c1["loglik"][0])
+    m.close()
+
+
+def test_nugget_loglik(lib):
+    g = np.load(os.path.join(GOLD, "nugget_900.npz"))
+    m = lib.ExactMLE(g["x"], g["y"], 64, kernel="univariate_matern_nuggets_stationary")
+    m.set_z(g["z"])
+    val = m.loglik(g["theta"])
+    assert abs(val - g["loglik"][0]) <= 1e-9 * abs(g["loglik"][0])
+    m.close()
+
+
+def test_bivariate_loglik(lib):
+    g = np.load(os.path.join(GOLD, "bivariate_400.npz"))
+    m = lib.ExactMLE(g["x"], g["y"], 64, kernel="bivariate_matern_parsimonious")
+    m.set_z(g["z"])
+    val = m.loglik(g["theta"])
+    assert abs(val - g["loglik"][0]) <= 1e-9 * abs(g["loglik"][0])
+    m.close()
+
+
+# ---------------------------------------------------------------- prediction
+def test_predict_c1_golden(lib, c1):
+    """Kriging of 100 held-out points: predictions within 1e-9 (relative to
+    ||zmiss||) and the MSE within 1e-9 relative."""
+    m = lib.ExactMLE(c1["x"], c1["y"], 320)
+    zm, mse = m.predict(c1["theta_true"], c1["xobs"], c1["yobs"], c1["zobs"], c1["xmiss"], c1["ymiss"],
+                        c1["zactual"])
+    assert np.linalg.norm(zm - c1["zmiss"]) <= 1e-9 * np.linalg.norm(c1["zmiss"])
+    assert abs(mse - c1["mse"][0]) <= 1e-9 * c1["mse"][0]
+    m.close()
+
+
+def test_predict_interpolates(lib, c1):
+    """Without a nugget, kriging at an observed location returns its value."""
+    m = lib.ExactMLE(c1["x"], c1["y"], 320)
+    idx = np.array([3, 500, 1200])
+    zm, _ = m.predict(c1["theta_true"], c1["x"], c1["y"], c1["z"], c1["x"][idx], c1["y"][idx])
+    assert np.max(np.abs(zm - c1["z"][idx])) <= 1e-8
+    m.close()
