@@ -102,6 +102,12 @@ _SIGNATURES = {
     "EXAGEOSTAT_Tile_to_Lapack": (ctypes.c_int, [_P(Desc), c_double_p, ctypes.c_int]),
     "EXAGEOSTAT_dpotrf_Tile": (ctypes.c_int, [ctypes.c_int, _P(Desc)]),
     "EXAGEOSTAT_dtrsv_Tile": (ctypes.c_int, [ctypes.c_int, _P(Desc), c_double_p]),
+    "exageostat_dlarnv": (None, [ctypes.c_int, _P(ctypes.c_int), ctypes.c_int, c_double_p]),
+    "exageostat_gpu_synchronize": (None, []),
+    "exageostat_gpu_profile_enable": (None, [ctypes.c_int]),
+    "exageostat_gpu_profile_reset": (None, []),
+    "exageostat_gpu_profile_get": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_long), c_double_p, c_double_p,
+                                                  c_double_p]),
     "exageostat_gpu_last_status": (ctypes.c_int, []),
     "exageostat_gpu_last_error": (ctypes.c_char_p, []),
     "exageostat_gpu_device_count": (ctypes.c_int, []),

@@ -18,6 +18,7 @@
 #include "engine.hpp"
 #include "mixed.hpp"
 #include "adjoint.hpp"
+#include "profile.hpp"
 
 using namespace exa;
 
@@ -858,6 +859,67 @@ double EXAGEOSTAT_dmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, i
 double MORSE_dmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
                                double* Zactual, double* Zmiss, int n) {
   return EXAGEOSTAT_dmle_Predict_Tile(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, n);
+}
+
+// ------------------------------------------------------- synthetic data --
+// LAPACK dlarnv: dlaruv's multiplicative congruential generator modulo 2^48
+// with multiplier a = 33952834046453 (u_i = s a^i mod 2^48 / 2^48 for
+// i = 1..m per call of at most 128 numbers), Box-Muller for idist = 3 in
+// chunks of 64.
+void exageostat_dlarnv(int idist, int* iseed, int n, double* x) {
+  const uint64_t mask = (1ull << 48) - 1;
+  const uint64_t a = 33952834046453ull;
+  uint64_t mult[128];
+  uint64_t p = 1;
+  for (int i = 0; i < 128; ++i) {
+    p = (uint64_t)(((unsigned __int128)p * a) & mask);
+    mult[i] = p;
+  }
+  uint64_t s = ((uint64_t)iseed[0] << 36) | ((uint64_t)iseed[1] << 24) | ((uint64_t)iseed[2] << 12) |
+               (uint64_t)iseed[3];
+  double u[128];
+  const double twopi = 6.28318530717958647692528676655900576839;
+  for (int iv = 0; iv < n; iv += 64) {
+    const int il = std::min(64, n - iv);
+    const int il2 = idist == 3 ? 2 * il : il;
+    for (int i = 0; i < il2; ++i)
+      u[i] = (double)(uint64_t)(((unsigned __int128)s * mult[i]) & mask) / 281474976710656.0;
+    s = (uint64_t)(((unsigned __int128)s * mult[il2 - 1]) & mask);
+    for (int i = 0; i < il; ++i) {
+      if (idist == 1) x[iv + i] = u[i];
+      else if (idist == 2) x[iv + i] = 2.0 * u[i] - 1.0;
+      else x[iv + i] = std::sqrt(-2.0 * std::log(u[2 * i])) * std::cos(twopi * u[2 * i + 1]);
+    }
+  }
+  iseed[0] = (int)((s >> 36) & 4095);
+  iseed[1] = (int)((s >> 24) & 4095);
+  iseed[2] = (int)((s >> 12) & 4095);
+  iseed[3] = (int)(s & 4095);
+}
+
+// ------------------------------------------------------------ measurement --
+void exageostat_gpu_synchronize(void) {
+  guarded([&]() {
+    EXA_HIP(hipDeviceSynchronize());
+    return 0;
+  }, 0);
+}
+
+void exageostat_gpu_profile_enable(int on) { Profiler::get().enable(on != 0); }
+
+void exageostat_gpu_profile_reset(void) { Profiler::get().reset(); }
+
+int exageostat_gpu_profile_get(const char* name, long* launches, double* ms, double* flops, double* bytes) {
+  return guarded([&]() {
+    EXA_HIP(hipDeviceSynchronize());
+    Profiler::get().collect();
+    KernelTotals t = Profiler::get().totals(name);
+    *launches = t.launches;
+    *ms = t.ms;
+    *flops = t.flops;
+    *bytes = t.bytes;
+    return 0;
+  }, -1);
 }
 
 }  // extern "C"

@@ -9,6 +9,8 @@
 #include "kernels.hpp"
 #include "mfma.hpp"
 
+#include <cstdlib>
+
 namespace exa {
 
 using C64 = GemmCfg<64, 64, 2, 2, 16>;
@@ -45,18 +47,23 @@ __global__ void __launch_bounds__(256) k_potrf_block(double* T, int ts, int j, d
     __syncthreads();
   }
   if (tid < 64) rdiag[tid] = 1.0 / L[tid * LD + tid];
+  // X = L^-1 by row sweeps: X starts as I; at step rr row rr is final after
+  // scaling by 1/L[rr][rr], then X[i][c] -= L[i][rr] X[rr][c] for i > rr,
+  // c <= rr (all threads in parallel).
+  for (int c = g; c < 64; c += 4) Inv[c * LD + r] = (r == c) ? 1.0 : 0.0;
   __syncthreads();
-  if (tid < 64) {
-    const int c = tid;
-    for (int rr = 0; rr < c; ++rr) Inv[c * LD + rr] = 0.0;
-    Inv[c * LD + c] = rdiag[c];
-    for (int rr = c + 1; rr < 64; ++rr) {
-      double s = 0.0;
-      for (int k = c; k < rr; ++k) s += L[k * LD + rr] * Inv[c * LD + k];
-      Inv[c * LD + rr] = -s * rdiag[rr];
+  for (int rr = 0; rr < 64; ++rr) {
+    if (tid <= rr) Inv[tid * LD + rr] *= rdiag[rr];
+    __syncthreads();
+    const int w = rr + 1;
+    const int cnt = (63 - rr) * w;
+    for (int e = tid; e < cnt; e += 256) {
+      const int i = rr + 1 + e / w;
+      const int c = e - (e / w) * w;
+      Inv[c * LD + i] -= L[rr * LD + i] * Inv[c * LD + rr];
     }
+    __syncthreads();
   }
-  __syncthreads();
   for (int c = g; c < 64; c += 4) {
     blk[r + (long)c * ts] = (r >= c) ? L[c * LD + r] : 0.0;
     dinv[r + c * 64] = Inv[c * LD + r];
@@ -117,8 +124,10 @@ __device__ void trsm_slab(double* X, long ldx, const double* L, long ldl, int nL
         for (int jj = 0; jj < C64::FN; ++jj) x[i][jj] = mfma_f64(a[i], b[jj], x[i][jj]);
     }
     acc_update<C64>(x, Xj, ldx, 0.0);
-    // X_jb is read back (from global memory) by the next column block.
-    __threadfence();
+    // X_jb is read back from global memory by the next column block of this
+    // same work-group: the barrier's workgroup-scope release/acquire makes
+    // the stores visible inside the CU (no device-scope fence, which would
+    // write back the XCD's L2).
     __syncthreads();
   }
 }
@@ -160,15 +169,21 @@ __global__ void __launch_bounds__(256) k_syrk_intra(double* T, int ts, int j) {
 }
 
 // ---------------------------------------------------------------------------
-// Trailing update A(m,n) -= A(m,k) A(n,k)^T, one work-group per BM x BN block.
+// Trailing update A(m,n) -= A(m,k) A(n,k)^T over BM x BN blocks.
 // col_lo == col_hi: the single tile column col_lo (tiles m >= col_lo);
 // otherwise the whole trailing triangle from column col_lo.
+//
+// Persistent launch: gridDim.x work-groups (at most one per CU, fewer than
+// the CU count so the panel kernels of the critical path always find free
+// CUs) loop over the block list.  Work-groups are dispatched round-robin
+// over the XCDs, so work-group w runs on XCD w % nx; each XCD gets a
+// contiguous slice of the block list and its work-groups walk it together,
+// which keeps the A/B row panels they share hot in that XCD's L2.
 template <class C>
-__global__ void __launch_bounds__(C::NT) k_tile_update(TileView v, int k, int col_lo, int col_hi) {
-  __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
+__device__ __forceinline__ void update_block(const TileView& v, int k, int col_lo, int col_hi, long b,
+                                             double* lds) {
   const int qm = v.ts / C::BM;
   const int qn = v.ts / C::BN;
-  const long b = blockIdx.x;
   const long p = b / (qm * qn);
   const int rem = (int)(b - p * (qm * qn));
   const int bi = rem % qm;
@@ -193,6 +208,72 @@ __global__ void __launch_bounds__(C::NT) k_tile_update(TileView v, int k, int co
   acc_update<C>(acc, Cp, v.ts, 1.0);
 }
 
+template <class C>
+__global__ void __launch_bounds__(C::NT) k_tile_update(TileView v, int k, int col_lo, int col_hi, long nblocks,
+                                                      int nx) {
+  __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
+  const int w = blockIdx.x;
+  const int x = w % nx;
+  const int gx = gridDim.x / nx;
+  const long per = (nblocks + nx - 1) / nx;
+  const long start = (long)x * per;
+  const long end = start + per < nblocks ? start + per : nblocks;
+  for (long b = start + w / nx; b < end; b += gx) update_block<C>(v, k, col_lo, col_hi, b, lds);
+}
+
+static int g_update_wgs = 0;
+
+// Work-groups of the persistent update: CUs minus a reserve for the panel
+// factorisation (EXAGEOSTAT_RESERVE_CUS, default 16), times the occupancy,
+// rounded down to a multiple of 8 (XCDs).
+template <class C>
+static int update_grid_cap() {
+  static int cap = 0;
+  if (cap) return cap;
+  int dev = 0, cus = 0, occ = 1;
+  EXA_HIP(hipGetDevice(&dev));
+  EXA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  EXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_tile_update<C>, C::NT, 0));
+  int reserve = 16;
+  if (const char* e = getenv("EXAGEOSTAT_RESERVE_CUS")) reserve = atoi(e);
+  if (reserve < 0 || reserve >= cus) reserve = 0;
+  cap = (cus - reserve) * (occ > 0 ? occ : 1);
+  if (cap >= 8) cap -= cap % 8;
+  if (cap < 1) cap = 1;
+  return cap;
+}
+
+template <class C>
+static void launch_update_cfg(const TileView& v, int k, int col_lo, int col_hi, hipStream_t s) {
+  long pairs;
+  if (col_lo == col_hi) {
+    pairs = v.mt - col_lo;
+  } else {
+    const long M = v.mt - col_lo;
+    pairs = M * (M + 1) / 2;
+  }
+  if (pairs <= 0) return;
+  const long blocks = pairs * (v.ts / C::BM) * (v.ts / C::BN);
+  const int cap = update_grid_cap<C>();
+  int grid = blocks < cap ? (int)blocks : cap;
+  int nx = 8;
+  if (grid % 8 != 0) nx = 1;
+  g_update_wgs = grid;
+  hipLaunchKernelGGL(k_tile_update<C>, dim3((unsigned)grid), dim3(C::NT), 0, s, v, k, col_lo, col_hi, blocks, nx);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_tile_update(const TileView& v, int k, int col_lo, int col_hi, hipStream_t s) {
+  if (col_lo >= v.mt || col_hi < col_lo) return;
+  if (v.ts % 192 == 0) {
+    launch_update_cfg<GemmCfg<192, 192, 2, 4, 16>>(v, k, col_lo, col_hi, s);
+  } else if (v.ts % 128 == 0) {
+    launch_update_cfg<GemmCfg<128, 128, 2, 2, 16>>(v, k, col_lo, col_hi, s);
+  } else {
+    launch_update_cfg<C64>(v, k, col_lo, col_hi, s);
+  }
+}
+
 // ---------------------------------------------------------------------------
 void launch_potrf_tile(double* T, int ts, double* dinv_k, int* info, long col_base, hipStream_t s) {
   const int q = ts / kNB;
@@ -214,32 +295,6 @@ void launch_trsm_panel(const TileView& v, int k, const double* dinv_k, hipStream
   if (slabs <= 0) return;
   hipLaunchKernelGGL(k_trsm_panel, dim3((unsigned)slabs), dim3(256), 0, s, v, k, dinv_k);
   EXA_HIP(hipGetLastError());
-}
-
-template <class C>
-static void launch_update_cfg(const TileView& v, int k, int col_lo, int col_hi, hipStream_t s) {
-  long pairs;
-  if (col_lo == col_hi) {
-    pairs = v.mt - col_lo;
-  } else {
-    const long M = v.mt - col_lo;
-    pairs = M * (M + 1) / 2;
-  }
-  if (pairs <= 0) return;
-  const long blocks = pairs * (v.ts / C::BM) * (v.ts / C::BN);
-  hipLaunchKernelGGL(k_tile_update<C>, dim3((unsigned)blocks), dim3(C::NT), 0, s, v, k, col_lo, col_hi);
-  EXA_HIP(hipGetLastError());
-}
-
-void launch_tile_update(const TileView& v, int k, int col_lo, int col_hi, hipStream_t s) {
-  if (col_lo >= v.mt || col_hi < col_lo) return;
-  if (v.ts % 192 == 0) {
-    launch_update_cfg<GemmCfg<192, 192, 2, 4, 16>>(v, k, col_lo, col_hi, s);
-  } else if (v.ts % 128 == 0) {
-    launch_update_cfg<GemmCfg<128, 128, 2, 2, 16>>(v, k, col_lo, col_hi, s);
-  } else {
-    launch_update_cfg<C64>(v, k, col_lo, col_hi, s);
-  }
 }
 
 }  // namespace exa

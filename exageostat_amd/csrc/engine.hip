@@ -2,6 +2,10 @@
 // single-GPU tile Cholesky scheduler and the likelihood driver.
 #include "engine.hpp"
 
+#include <algorithm>
+
+#include "profile.hpp"
+
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -273,6 +277,27 @@ hipEvent_t CholWorkspace::ev(int i) {
   return ev_[i];
 }
 
+// Algorithmic flops of A(m,n) -= A(m,k) A(n,k)^T over n in [lo, hi],
+// m in [n, mt) on the logical (unpadded) sizes: 2 r_m r_n K per
+// off-diagonal tile, r_n (r_n + 1) K per diagonal tile (FLOPS_DSYRK).
+static double update_flops(const TileMatrix& A, int k, int lo, int hi) {
+  auto rows = [&](int t) { return (double)std::min<long>(A.ts, (long)A.m - (long)t * A.ts); };
+  const double K = rows(k);
+  double f = 0.0;
+  for (int n = lo; n <= hi && n < A.mt; ++n) {
+    const double rn = rows(n);
+    f += rn * (rn + 1.0) * K;
+    double below = 0.0;
+    for (int m = n + 1; m < A.mt; ++m) below += rows(m);
+    f += 2.0 * below * rn * K;
+  }
+  return f;
+}
+
+static double potrf_flops(double n) {
+  return n * (((1.0 / 6.0) * n + 0.5) * n + 1.0 / 3.0) + n * (((1.0 / 6.0) * n) * n - 1.0 / 6.0);
+}
+
 int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2, bool sync) {
   const int mt = A.mt;
   const int ts = A.ts;
@@ -280,32 +305,43 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
   int* d_info = ws.info();
   EXA_HIP(hipMemsetAsync(d_info, 0, sizeof(int), s1));
   TileView v = A.view();
+  auto rows = [&](int t) { return (double)std::min<long>(ts, (long)A.m - (long)t * ts); };
+  auto factor_panel = [&](int k) {
+    const double rk = rows(k);
+    profiled("potrf_diag", potrf_flops(rk), 0.0, s1,
+             [&] { launch_potrf_tile(A.tile(k, k), ts, ws.dinv(k), d_info, (long)k * ts, s1); });
+    const double below = (double)A.m - (double)(k + 1) * ts;
+    if (k + 1 < mt)
+      profiled("trsm_panel", below * rk * rk, 0.0, s1, [&] { launch_trsm_panel(v, k, ws.dinv(k), s1); });
+  };
+  auto update = [&](int k, int lo, int hi, hipStream_t s) {
+    profiled("tile_update", update_flops(A, k, lo, hi), 0.0, s, [&] { launch_tile_update(v, k, lo, hi, s); });
+  };
+  // s1 carries the panel factorisations (small kernels on the CUs the
+  // persistent update leaves free); s2 carries the trailing updates: first
+  // column k+1 of step k (which unblocks panel k+1), then the rest of step k,
+  // which overlaps with panel k+1 on s1.
   EXA_HIP(hipEventRecord(ws.ev(0), s1));
   EXA_HIP(hipStreamWaitEvent(s2, ws.ev(0), 0));
-  launch_potrf_tile(A.tile(0, 0), ts, ws.dinv(0), d_info, 0, s1);
-  launch_trsm_panel(v, 0, ws.dinv(0), s1);
+  factor_panel(0);
   EXA_HIP(hipEventRecord(ws.ev(1), s1));
   for (int k = 0; k + 1 < mt; ++k) {
     hipEvent_t ev_panel_k = ws.ev(1 + 2 * k);
     hipEvent_t ev_panel_k1 = ws.ev(1 + 2 * (k + 1));
-    hipEvent_t ev_rest_k = ws.ev(2 + 2 * k);
-    // Critical path: column k+1 by step k, then factor panel k+1.
-    launch_tile_update(v, k, k + 1, k + 1, s1);
-    launch_potrf_tile(A.tile(k + 1, k + 1), ts, ws.dinv(k + 1), d_info, (long)(k + 1) * ts, s1);
-    launch_trsm_panel(v, k + 1, ws.dinv(k + 1), s1);
+    hipEvent_t ev_col_k = ws.ev(2 + 2 * k);
+    EXA_HIP(hipStreamWaitEvent(s2, ev_panel_k, 0));
+    update(k, k + 1, k + 1, s2);
+    EXA_HIP(hipEventRecord(ev_col_k, s2));
+    if (k + 2 < mt) update(k, k + 2, mt - 1, s2);
+    EXA_HIP(hipStreamWaitEvent(s1, ev_col_k, 0));
+    factor_panel(k + 1);
     EXA_HIP(hipEventRecord(ev_panel_k1, s1));
-    // Remaining trailing update of step k overlaps with the panel above.
-    if (k + 2 < mt) {
-      EXA_HIP(hipStreamWaitEvent(s2, ev_panel_k, 0));
-      launch_tile_update(v, k, k + 2, mt - 1, s2);
-    }
-    EXA_HIP(hipEventRecord(ev_rest_k, s2));
-    EXA_HIP(hipStreamWaitEvent(s1, ev_rest_k, 0));
   }
-  // s2 has no work after the last rest update; make s2 wait for s1 so later
-  // work on either stream sees the finished factor.
-  EXA_HIP(hipEventRecord(ws.ev(2 * mt + 2), s1));
-  EXA_HIP(hipStreamWaitEvent(s2, ws.ev(2 * mt + 2), 0));
+  // Join: s1 waits for the last updates on s2, s2 for the last panel on s1.
+  EXA_HIP(hipEventRecord(ws.ev(2 * mt + 2), s2));
+  EXA_HIP(hipStreamWaitEvent(s1, ws.ev(2 * mt + 2), 0));
+  EXA_HIP(hipEventRecord(ws.ev(2 * mt + 3), s1));
+  EXA_HIP(hipStreamWaitEvent(s2, ws.ev(2 * mt + 3), 0));
   launch_zero_upper_diag(v, s1);
   EXA_HIP(hipMemcpyAsync(ws.host_info(), d_info, sizeof(int), hipMemcpyDeviceToHost, s1));
   if (!sync) return 0;
@@ -381,7 +417,8 @@ double Context::loglik(KernelId kid, const double* theta, int metric, double* lo
                        int* info) {
   const double t0 = wall_seconds();
   EXA_HIP(hipMemcpyAsync(work.d, Zcpy.d, sizeof(double) * work.padded, hipMemcpyDeviceToDevice, s1));
-  generate_covariance(C, kid, theta, l1, l1, metric, s1);
+  profiled("gen", 0.0, 4.0 * (double)N * (N + 1.0), s1,
+           [&] { generate_covariance(C, kid, theta, l1, l1, metric, s1); });
   EXA_HIP(hipStreamSynchronize(s1));
   const double t1 = wall_seconds();
   const int inf = tile_potrf(C, ws, s1, s2, true);
@@ -389,9 +426,11 @@ double Context::loglik(KernelId kid, const double* theta, int metric, double* lo
   if (info) *info = inf;
   const double logdet = 2.0 * tile_logdet(C, d_partial, s1);
   const double t3 = wall_seconds();
-  tile_trsv(C, ws, work.d, 0, s1);
+  profiled("trsv", (double)N * (N + 1.0), 4.0 * (double)N * (N + 1.0), s1,
+           [&] { tile_trsv(C, ws, work.d, 0, s1); });
   const double dotp = device_dot(work.d, work.d, N, d_partial, s1);
   const double t4 = wall_seconds();
+  if (Profiler::get().enabled()) Profiler::get().collect();
   last.gen = t1 - t0;
   last.facto = t2 - t1;
   last.logdet = t3 - t2;

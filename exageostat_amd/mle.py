@@ -50,6 +50,36 @@ def generate_xy_loc(n: int, seed: int):
     return x, y
 
 
+def dlarnv(idist: int, iseed, n: int):
+    """LAPACK dlarnv through the library; returns (values, advanced iseed)."""
+    lib = _lib.load()
+    seed = (ctypes.c_int * 4)(*[int(v) for v in iseed])
+    out = np.empty(n)
+    lib.exageostat_dlarnv(idist, seed, n, _dp(out))
+    return out, list(seed)
+
+
+def synchronize() -> None:
+    _lib.load().exageostat_gpu_synchronize()
+
+
+def profile_enable(on: bool = True) -> None:
+    lib = _lib.load()
+    lib.exageostat_gpu_profile_reset()
+    lib.exageostat_gpu_profile_enable(1 if on else 0)
+
+
+def profile_get(name: str) -> dict:
+    lib = _lib.load()
+    launches = ctypes.c_long(0)
+    ms = ctypes.c_double(0)
+    fl = ctypes.c_double(0)
+    by = ctypes.c_double(0)
+    lib.exageostat_gpu_profile_get(name.encode(), ctypes.byref(launches), ctypes.byref(ms), ctypes.byref(fl),
+                                   ctypes.byref(by))
+    return {"launches": launches.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+
+
 class ExactMLE:
     """One MLE_data with its GPU context: EXAGEOSTAT_dmle_Call at
     construction, MLE_Finalize at close()."""

@@ -223,6 +223,19 @@ int EXAGEOSTAT_dpotrf_Tile(CHAM_enum uplo, CHAM_desc_t *A);
  * given in host memory; L from EXAGEOSTAT_dpotrf_Tile. */
 int EXAGEOSTAT_dtrsv_Tile(CHAM_enum trans, CHAM_desc_t *L, double *b);
 
+/* LAPACK dlarnv (the reference draws N(0,1) with LAPACKE_dlarnv(3, iseed, N,
+ * Nrand), examples/synthetic_dmle_test.c:111,147); iseed is advanced. */
+void exageostat_dlarnv(int idist, int *iseed, int n, double *x);
+
+/* ------------------------------------------------------------ measurement -- */
+/* No counterpart in the reference.  Per-kernel HIP-event timing on the
+ * launching stream ("tile_update", "potrf_diag", "trsm_panel", "gen", "trsv"),
+ * used by the benchmark; off by default. */
+void exageostat_gpu_synchronize(void);
+void exageostat_gpu_profile_enable(int on);
+void exageostat_gpu_profile_reset(void);
+int exageostat_gpu_profile_get(const char *name, long *launches, double *ms, double *flops, double *bytes);
+
 /* ------------------------------------------------------------- status -- */
 int exageostat_gpu_last_status(void);
 const char *exageostat_gpu_last_error(void);

