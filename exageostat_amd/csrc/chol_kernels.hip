@@ -169,6 +169,87 @@ __global__ void __launch_bounds__(256) k_syrk_intra(double* T, int ts, int j) {
 }
 
 // ---------------------------------------------------------------------------
+// Explicit inverse X = L_kk^-1 of every diagonal tile (for the triangular
+// solves, which then become matrix-vector products).  One work-group per
+// (tile k, 64-column block j): X_jj = inv(L_jj) from the potrf, then for
+// i > j: X_ij = -inv(L_ii) sum_{l=j}^{i-1} L_il X_lj.
+__global__ void __launch_bounds__(256) k_trtri_tiles(TileView v, const double* dinv, double* xinv) {
+  __shared__ __attribute__((aligned(16))) double lds[2 * 64 * 68];
+  constexpr int LDT = 68;
+  const int ts = v.ts;
+  const int q = ts / kNB;
+  const int k = blockIdx.x / q;
+  const int j = blockIdx.x - k * q;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave / C64::WGN;
+  const int wn = wave - wm * C64::WGN;
+  const double* L = tile_f64(v, k, k);
+  double* X = xinv + (long)k * ts * ts;
+  const double* dk = dinv + (long)k * q * 4096;
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e & 63, c = e >> 6;
+    X[(j * kNB + r) + (long)(j * kNB + c) * ts] = dk[(long)j * 4096 + r + c * 64];
+  }
+  for (int e = tid; e < j * kNB * kNB; e += 256) {
+    const int r = e % (j * kNB), c = e / (j * kNB);
+    X[r + (long)(j * kNB + c) * ts] = 0.0;
+  }
+  __syncthreads();
+  for (int i = j + 1; i < q; ++i) {
+    dx4 acc[C64::FM][C64::FN];
+    zero_acc<C64>(acc);
+    gemm_nt_acc<C64, true>(acc, L + i * kNB + (long)j * kNB * ts, ts, X + j * kNB + (long)j * kNB * ts, ts,
+                           (i - j) * kNB, lds, 1.0);
+    double* As = lds;             // inv(L_ii): As[k][r] = inv(r, k)
+    double* Bs = lds + 64 * LDT;  // acc:       Bs[k][c] = acc(k, c)
+#pragma unroll
+    for (int a = 0; a < C64::FM; ++a)
+#pragma unroll
+      for (int b = 0; b < C64::FN; ++b) {
+        const int c = wn * C64::WN + b * 16 + frag_col(lane);
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) {
+          const int r = wm * C64::WM + a * 16 + frag_row(lane, vv);
+          Bs[r * LDT + c] = acc[a][b][vv];
+        }
+      }
+    const double* inv = dk + (long)i * 4096;
+    for (int e = tid; e < 64 * 32; e += 256) {
+      const int kk = e >> 5;
+      const int r2 = e & 31;
+      *reinterpret_cast<double2*>(As + kk * LDT + 2 * r2) = *reinterpret_cast<const double2*>(inv + 2 * r2 + kk * 64);
+    }
+    __syncthreads();
+    dx4 y[C64::FM][C64::FN];
+    zero_acc<C64>(y);
+#pragma unroll 4
+    for (int kk = 0; kk < 64; kk += 4) {
+      const int krow = kk + (lane >> 4);
+      double a[C64::FM], b[C64::FN];
+#pragma unroll
+      for (int ii = 0; ii < C64::FM; ++ii) a[ii] = -As[krow * LDT + wm * C64::WM + ii * 16 + (lane & 15)];
+#pragma unroll
+      for (int jj = 0; jj < C64::FN; ++jj) b[jj] = Bs[krow * LDT + wn * C64::WN + jj * 16 + (lane & 15)];
+#pragma unroll
+      for (int ii = 0; ii < C64::FM; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < C64::FN; ++jj) y[ii][jj] = mfma_f64(a[ii], b[jj], y[ii][jj]);
+    }
+    acc_update<C64>(y, X + i * kNB + (long)j * kNB * ts, ts, 0.0);
+    // X_ij is read back by the next i of this work-group (same CU).
+    __syncthreads();
+  }
+}
+
+void launch_trtri_tiles(const TileView& v, const double* dinv, double* xinv, hipStream_t s) {
+  const int q = v.ts / kNB;
+  hipLaunchKernelGGL(k_trtri_tiles, dim3((unsigned)(v.mt * q)), dim3(256), 0, s, v, dinv, xinv);
+  EXA_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // Trailing update A(m,n) -= A(m,k) A(n,k)^T over BM x BN blocks.
 // col_lo == col_hi: the single tile column col_lo (tiles m >= col_lo);
 // otherwise the whole trailing triangle from column col_lo.

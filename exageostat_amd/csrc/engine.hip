@@ -252,6 +252,8 @@ void generate_covariance(TileMatrix& A, KernelId kid, const double* theta, const
 // --------------------------------------------------------------- Cholesky --
 CholWorkspace::~CholWorkspace() {
   if (dinv_) (void)hipFree(dinv_);
+  if (xinv_) (void)hipFree(xinv_);
+  if (yvec_) (void)hipFree(yvec_);
   if (info_) (void)hipFree(info_);
   if (h_info_) (void)hipHostFree(h_info_);
   for (auto e : ev_) (void)hipEventDestroy(e);
@@ -260,10 +262,14 @@ CholWorkspace::~CholWorkspace() {
 void CholWorkspace::ensure(int mt, int ts) {
   if (dinv_ && mt <= mt_ && ts == ts_) return;
   if (dinv_) (void)hipFree(dinv_);
-  dinv_ = nullptr;
+  if (xinv_) (void)hipFree(xinv_);
+  if (yvec_) (void)hipFree(yvec_);
+  dinv_ = xinv_ = yvec_ = nullptr;
   mt_ = mt;
   ts_ = ts;
   EXA_HIP(hipMalloc(&dinv_, sizeof(double) * (size_t)mt * (ts / kNB) * 4096));
+  EXA_HIP(hipMalloc(&xinv_, sizeof(double) * (size_t)mt * ts * ts));
+  EXA_HIP(hipMalloc(&yvec_, sizeof(double) * (size_t)mt * ts));
   if (!info_) EXA_HIP(hipMalloc(&info_, sizeof(int)));
   if (!h_info_) EXA_HIP(hipHostMalloc(&h_info_, sizeof(int), hipHostMallocDefault));
 }
@@ -343,6 +349,8 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
   EXA_HIP(hipEventRecord(ws.ev(2 * mt + 3), s1));
   EXA_HIP(hipStreamWaitEvent(s2, ws.ev(2 * mt + 3), 0));
   launch_zero_upper_diag(v, s1);
+  profiled("trtri_diag", (double)mt * ts * (double)ts * ts / 3.0, 0.0, s1,
+           [&] { launch_trtri_tiles(v, ws.dinv_base(), ws.xinv_base(), s1); });
   EXA_HIP(hipMemcpyAsync(ws.host_info(), d_info, sizeof(int), hipMemcpyDeviceToHost, s1));
   if (!sync) return 0;
   EXA_HIP(hipStreamSynchronize(s1));
@@ -350,18 +358,24 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
 }
 
 void tile_trsv(const TileMatrix& L, const CholWorkspace& ws, double* z, int trans, hipStream_t s) {
+  // Blocked substitution with the inverted diagonal tiles: y_k = X_k z_k
+  // (resp. X_k^T z_k), then the off-diagonal tiles update the remaining
+  // right-hand side; y is assembled in a scratch vector and copied back.
   TileView v = L.view();
+  const int ts = L.ts;
+  double* y = ws.yvec();
   if (!trans) {
     for (int k = 0; k < L.mt; ++k) {
-      launch_trsv_tile(L.tile(k, k), L.ts, ws.dinv(k), z + (long)k * L.ts, 0, s);
-      launch_gemv_panel(v, k, z, s);
+      launch_trmv_diag(ws.xinv(k), ts, z + (long)k * ts, y + (long)k * ts, 0, s);
+      launch_gemv_panel(v, k, y + (long)k * ts, z, s);
     }
   } else {
     for (int k = L.mt - 1; k >= 0; --k) {
-      launch_trsv_tile(L.tile(k, k), L.ts, ws.dinv(k), z + (long)k * L.ts, 1, s);
-      launch_gemv_t_row(v, k, z, s);
+      launch_trmv_diag(ws.xinv(k), ts, z + (long)k * ts, y + (long)k * ts, 1, s);
+      launch_gemv_t_row(v, k, y + (long)k * ts, z, s);
     }
   }
+  EXA_HIP(hipMemcpyAsync(z, y, sizeof(double) * (size_t)L.mt * ts, hipMemcpyDeviceToDevice, s));
 }
 
 double tile_logdet(const TileMatrix& L, double* d_partial, hipStream_t s) {

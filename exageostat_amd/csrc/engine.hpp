@@ -94,6 +94,12 @@ class CholWorkspace {
   ~CholWorkspace();
   void ensure(int mt, int ts);
   double* dinv(int k) const { return dinv_ + (long)k * (ts_ / kNB) * 4096; }
+  double* dinv_base() const { return dinv_; }
+  // Inverse of diagonal tile k (ts x ts, lower), filled by tile_potrf.
+  double* xinv(int k) const { return xinv_ + (long)k * ts_ * ts_; }
+  double* xinv_base() const { return xinv_; }
+  // Scratch vector of mt*ts doubles for the triangular solves.
+  double* yvec() const { return yvec_; }
   int* info() const { return info_; }      // device
   int* host_info() const { return h_info_; }  // pinned host mirror
   hipEvent_t ev(int i);
@@ -101,6 +107,8 @@ class CholWorkspace {
 
  private:
   double* dinv_ = nullptr;
+  double* xinv_ = nullptr;
+  double* yvec_ = nullptr;
   int* info_ = nullptr;
   int* h_info_ = nullptr;
   std::vector<hipEvent_t> ev_;

@@ -49,12 +49,16 @@ void launch_trsm_panel(const TileView& v, int k, const double* dinv_k, hipStream
 // A(m,n) -= A(m,k) A(n,k)^T for n in [col_lo, col_hi], m in [n, mt).
 void launch_tile_update(const TileView& v, int k, int col_lo, int col_hi, hipStream_t s);
 
+// X_k = L_kk^-1 for every diagonal tile (xinv: mt tiles of ts x ts).
+void launch_trtri_tiles(const TileView& v, const double* dinv, double* xinv, hipStream_t s);
+
 // ---- vectors --------------------------------------------------------------
-// z_k <- L_kk^-1 z_k (single tile), then z_m -= L(m,k) z_k for m > k.
-void launch_trsv_tile(const double* T, int ts, const double* dinv_k, double* zk, int trans, hipStream_t s);
-void launch_gemv_panel(const TileView& v, int k, double* z, hipStream_t s);
-// z_m -= L(k,m)^T z_k for m < k.
-void launch_gemv_t_row(const TileView& v, int k, double* z, hipStream_t s);
+// y = X t (trans = 0) or X^T t (trans = 1) for one lower ts x ts tile X.
+void launch_trmv_diag(const double* X, int ts, const double* t, double* y, int trans, hipStream_t s);
+// z_m -= L(m,k) y_k for m in (k, mt).
+void launch_gemv_panel(const TileView& v, int k, const double* yk, double* z, hipStream_t s);
+// z_m -= L(k,m)^T y_k for m in [0, k).
+void launch_gemv_t_row(const TileView& v, int k, const double* yk, double* z, hipStream_t s);
 // partial[k] = sum_i log(L_kk[i,i]) over positive pivots of each diagonal tile.
 void launch_logdet(const TileView& v, double* partial, hipStream_t s);
 // partial[b] = sum of z[i]*w[i] over 4096-element blocks.

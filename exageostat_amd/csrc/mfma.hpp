@@ -9,6 +9,8 @@
 
 #include "common.hpp"
 
+#include <cstdint>
+
 namespace exa {
 
 typedef double dx4 __attribute__((ext_vector_type(4)));
@@ -42,6 +44,18 @@ struct GemmCfg {
   static_assert(B_LOADS * NT * 2 == BN * BK, "B slice must split evenly");
 };
 
+// Loads through an explicit global (address space 1) pointer: the tile
+// pointers come from a table, so without the cast the compiler emits flat
+// loads whose completion it can only wait for together with LDS traffic.
+__device__ __forceinline__ double2 ld_global2(const double* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(1))) const double2 gdouble2;
+  return *reinterpret_cast<gdouble2*>(reinterpret_cast<uintptr_t>(p));
+#else
+  return *reinterpret_cast<const double2*>(p);
+#endif
+}
+
 template <class C>
 __device__ __forceinline__ void zero_acc(dx4 (&acc)[C::FM][C::FN]) {
 #pragma unroll
@@ -51,8 +65,9 @@ __device__ __forceinline__ void zero_acc(dx4 (&acc)[C::FM][C::FN]) {
 }
 
 // K must be a multiple of BK; lda/ldb even and A/B 16-byte aligned.
-// Ends with a barrier, so the caller may reuse the LDS afterwards.
-template <class C>
+// B_KMAJOR: B is given as K x BN column-major (B(k, c) at B[k + c*ldb])
+// instead of BN x K.  Ends with a barrier, so the caller may reuse the LDS.
+template <class C, bool B_KMAJOR = false>
 __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const double* __restrict__ A, long lda,
                                             const double* __restrict__ B, long ldb, int K, double* lds,
                                             double sign) {
@@ -73,14 +88,20 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const doub
       const int e = tid + t * C::NT;
       const int c = e / (C::BM / 2);
       const int r2 = e - c * (C::BM / 2);
-      ra[t] = *reinterpret_cast<const double2*>(A + (long)(k0 + c) * lda + 2 * r2);
+      ra[t] = ld_global2(A + (long)(k0 + c) * lda + 2 * r2);
     }
 #pragma unroll
     for (int t = 0; t < C::B_LOADS; ++t) {
       const int e = tid + t * C::NT;
-      const int c = e / (C::BN / 2);
-      const int r2 = e - c * (C::BN / 2);
-      rb[t] = *reinterpret_cast<const double2*>(B + (long)(k0 + c) * ldb + 2 * r2);
+      if (B_KMAJOR) {
+        const int c = e / (C::BK / 2);
+        const int k2 = e - c * (C::BK / 2);
+        rb[t] = ld_global2(B + (long)c * ldb + k0 + 2 * k2);
+      } else {
+        const int c = e / (C::BN / 2);
+        const int r2 = e - c * (C::BN / 2);
+        rb[t] = ld_global2(B + (long)(k0 + c) * ldb + 2 * r2);
+      }
     }
   };
   auto sstore = [&](int buf) {
@@ -94,9 +115,16 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const doub
 #pragma unroll
     for (int t = 0; t < C::B_LOADS; ++t) {
       const int e = tid + t * C::NT;
-      const int c = e / (C::BN / 2);
-      const int r2 = e - c * (C::BN / 2);
-      *reinterpret_cast<double2*>(Bs + buf * C::B_STAGE + c * C::LDB + 2 * r2) = rb[t];
+      if (B_KMAJOR) {
+        const int c = e / (C::BK / 2);
+        const int k2 = e - c * (C::BK / 2);
+        Bs[buf * C::B_STAGE + (2 * k2) * C::LDB + c] = rb[t].x;
+        Bs[buf * C::B_STAGE + (2 * k2 + 1) * C::LDB + c] = rb[t].y;
+      } else {
+        const int c = e / (C::BN / 2);
+        const int r2 = e - c * (C::BN / 2);
+        *reinterpret_cast<double2*>(Bs + buf * C::B_STAGE + c * C::LDB + 2 * r2) = rb[t];
+      }
     }
   };
 
@@ -122,6 +150,10 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const doub
 #pragma unroll
         for (int j = 0; j < C::FN; ++j) acc[i][j] = mfma_f64(a[i], b[j], acc[i][j]);
     }
+    // Keep the LDS stores of the prefetched slice after the matrix-core work
+    // (otherwise they are hoisted next to the global loads and every slice
+    // waits for memory before its MFMAs).
+    __builtin_amdgcn_sched_barrier(0);
     if (kc + 1 < nk) sstore(buf ^ 1);
     __syncthreads();
   }

@@ -19,66 +19,44 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Single tile: z <- L^-1 z (trans = 0) or z <- L^-T z (trans = 1), L lower
-// ts x ts (column-major, ld = ts) with inverses of its 64x64 diagonal blocks
-// in dinv.  One work-group; z lives in LDS during the sweep.
-__global__ void __launch_bounds__(256) k_trsv_tile(const double* T, int ts, const double* dinv, double* z,
+// y = X t (trans = 0) or y = X^T t (trans = 1) for one lower-triangular
+// ts x ts tile X (the inverse of a diagonal factor tile); one work-group
+// per 64 rows (trans = 0) or 64 columns (trans = 1).
+__global__ void __launch_bounds__(256) k_trmv_diag(const double* X, int ts, const double* t, double* y,
                                                    int trans) {
-  __shared__ double y[1024];
   __shared__ double part[4][64];
-  __shared__ double t[64];
   const int tid = threadIdx.x;
   const int r = tid & 63;
   const int g = tid >> 6;
-  const int q = ts / kNB;
-  for (int i = tid; i < ts; i += 256) y[i] = z[i];
-  __syncthreads();
+  const int b = blockIdx.x;
   if (!trans) {
-    for (int j = 0; j < q; ++j) {
-      const int row = j * kNB + r;
-      double s = 0.0;
-      for (int c = g; c < j * kNB; c += 4) s += T[row + (long)c * ts] * y[c];
-      part[g][r] = s;
-      __syncthreads();
-      if (tid < 64) t[r] = y[row] - ((part[0][r] + part[1][r]) + (part[2][r] + part[3][r]));
-      __syncthreads();
-      if (tid < 64) {
-        const double* inv = dinv + (long)j * 4096;
-        double acc = 0.0;
-        for (int c = 0; c <= r; ++c) acc += inv[r + c * 64] * t[c];
-        y[row] = acc;
-      }
-      __syncthreads();
+    const int row = b * kNB + r;
+    const int cend = (b + 1) * kNB;
+    double s0 = 0.0, s1 = 0.0;
+    int c = g;
+    for (; c + 4 < cend; c += 8) {
+      s0 += X[row + (long)c * ts] * t[c];
+      s1 += X[row + (long)(c + 4) * ts] * t[c + 4];
     }
+    for (; c < cend; c += 4) s0 += X[row + (long)c * ts] * t[c];
+    part[g][r] = s0 + s1;
+    __syncthreads();
+    if (tid < 64) y[row] = (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
   } else {
-    for (int j = q - 1; j >= 0; --j) {
-      // s[col] = sum_{rows below block j} L[row][col] y[row] for the 64
-      // columns col of block j; each wave owns 16 columns, lanes run rows.
-      for (int cc = g * 16; cc < g * 16 + 16; ++cc) {
-        const int col = j * kNB + cc;
-        double s = 0.0;
-        for (int row = (j + 1) * kNB + r; row < ts; row += 64) s += T[row + (long)col * ts] * y[row];
-        s = wave_sum(s);
-        if (r == 0) part[0][cc] = s;
-      }
-      __syncthreads();
-      if (tid < 64) t[r] = y[j * kNB + r] - part[0][r];
-      __syncthreads();
-      if (tid < 64) {
-        // y_j = inv^T t: y[r] = sum_c inv[c][r] t[c] = sum_{c >= r} dinv[c + r*64] t[c]
-        const double* inv = dinv + (long)j * 4096;
-        double acc = 0.0;
-        for (int c = r; c < 64; ++c) acc += inv[c + r * 64] * t[c];
-        y[j * kNB + r] = acc;
-      }
-      __syncthreads();
+    for (int cc = g * 16; cc < g * 16 + 16; ++cc) {
+      const int col = b * kNB + cc;
+      double s = 0.0;
+      for (int row = b * kNB + r; row < ts; row += 64) s += X[row + (long)col * ts] * t[row];
+      s = wave_sum(s);
+      if (r == 0) part[0][cc] = s;
     }
+    __syncthreads();
+    if (tid < 64) y[b * kNB + r] = part[0][r];
   }
-  for (int i = tid; i < ts; i += 256) z[i] = y[i];
 }
 
-// z_m -= L(m,k) z_k for m in (k, mt); one work-group per 64-row block.
-__global__ void __launch_bounds__(256) k_gemv_panel(TileView v, int k, double* z) {
+// z_m -= L(m,k) y_k for m in (k, mt); one work-group per 64-row block.
+__global__ void __launch_bounds__(256) k_gemv_panel(TileView v, int k, const double* yk, double* z) {
   __shared__ double y[1024];
   __shared__ double part[4][64];
   const int tid = threadIdx.x;
@@ -88,8 +66,7 @@ __global__ void __launch_bounds__(256) k_gemv_panel(TileView v, int k, double* z
   const int q = ts / kNB;
   const int m = k + 1 + blockIdx.x / q;
   const int rb = blockIdx.x - (blockIdx.x / q) * q;
-  const double* zk = z + (long)k * ts;
-  for (int i = tid; i < ts; i += 256) y[i] = zk[i];
+  for (int i = tid; i < ts; i += 256) y[i] = yk[i];
   __syncthreads();
   const double* A = tile_f64(v, m, k) + rb * kNB + r;
   double s0 = 0.0, s1 = 0.0;
@@ -106,8 +83,8 @@ __global__ void __launch_bounds__(256) k_gemv_panel(TileView v, int k, double* z
   }
 }
 
-// z_m -= L(k,m)^T z_k for m in [0, k); one work-group per 64 columns of L(k,m).
-__global__ void __launch_bounds__(256) k_gemv_t_row(TileView v, int k, double* z) {
+// z_m -= L(k,m)^T y_k for m in [0, k); one work-group per 64 columns of L(k,m).
+__global__ void __launch_bounds__(256) k_gemv_t_row(TileView v, int k, const double* yk, double* z) {
   __shared__ double y[1024];
   __shared__ double part[64];
   const int tid = threadIdx.x;
@@ -117,8 +94,7 @@ __global__ void __launch_bounds__(256) k_gemv_t_row(TileView v, int k, double* z
   const int q = ts / kNB;
   const int m = blockIdx.x / q;
   const int cb = blockIdx.x - m * q;
-  const double* zk = z + (long)k * ts;
-  for (int i = tid; i < ts; i += 256) y[i] = zk[i];
+  for (int i = tid; i < ts; i += 256) y[i] = yk[i];
   __syncthreads();
   const double* A = tile_f64(v, k, m);
   for (int cc = g * 16; cc < g * 16 + 16; ++cc) {
@@ -214,24 +190,24 @@ void launch_zero_upper_diag(const TileView& v, hipStream_t s) {
   EXA_HIP(hipGetLastError());
 }
 
-void launch_trsv_tile(const double* T, int ts, const double* dinv_k, double* zk, int trans, hipStream_t s) {
-  hipLaunchKernelGGL(k_trsv_tile, dim3(1), dim3(256), 0, s, T, ts, dinv_k, zk, trans);
+void launch_trmv_diag(const double* X, int ts, const double* t, double* y, int trans, hipStream_t s) {
+  hipLaunchKernelGGL(k_trmv_diag, dim3(ts / kNB), dim3(256), 0, s, X, ts, t, y, trans);
   EXA_HIP(hipGetLastError());
 }
 
-void launch_gemv_panel(const TileView& v, int k, double* z, hipStream_t s) {
+void launch_gemv_panel(const TileView& v, int k, const double* yk, double* z, hipStream_t s) {
   const int q = v.ts / kNB;
   const long blocks = (long)(v.mt - k - 1) * q;
   if (blocks <= 0) return;
-  hipLaunchKernelGGL(k_gemv_panel, dim3((unsigned)blocks), dim3(256), 0, s, v, k, z);
+  hipLaunchKernelGGL(k_gemv_panel, dim3((unsigned)blocks), dim3(256), 0, s, v, k, yk, z);
   EXA_HIP(hipGetLastError());
 }
 
-void launch_gemv_t_row(const TileView& v, int k, double* z, hipStream_t s) {
+void launch_gemv_t_row(const TileView& v, int k, const double* yk, double* z, hipStream_t s) {
   const int q = v.ts / kNB;
   const long blocks = (long)k * q;
   if (blocks <= 0) return;
-  hipLaunchKernelGGL(k_gemv_t_row, dim3((unsigned)blocks), dim3(256), 0, s, v, k, z);
+  hipLaunchKernelGGL(k_gemv_t_row, dim3((unsigned)blocks), dim3(256), 0, s, v, k, yk, z);
   EXA_HIP(hipGetLastError());
 }
 
