@@ -77,6 +77,43 @@ def cpu_baseline(n_sample: int, n_target: int, theta, threads: int) -> dict:
     }
 
 
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix peak
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+
+# Mixed-precision options of the benchmark: tiles whose largest correlation
+# is below FP32_CORR get BF16 MFMA updates, the other off-diagonal tiles are
+# FP32, diagonal tiles FP64; 3 FP64 refinement steps on the solve.
+MIXED = dict(fp32_corr=1e-2, bf16=True, fp64_corr=0.0, diag_thick=1, ir_steps=3)
+
+
+def _timed(m, theta, steps: int, warmup: int, names):
+    from exageostat_amd import mle
+    for _ in range(warmup):
+        m.loglik(theta)
+    mle.synchronize()
+    mle.profile_enable(True)
+    phases = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ll = m.loglik(theta)
+        phases.append(m.phase_times())
+    mle.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = {k: mle.profile_get(k) for k in names}
+    mle.profile_enable(False)
+    return {
+        "elapsed": elapsed, "ll": ll, "status": m.status, "logdet": m.logdet, "dotp": m.dotp,
+        "phases": {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}, "prof": prof,
+    }
+
+
+def _rate(p, key):
+    """(achieved units/s, per-launch ms) of one profiled kernel group."""
+    if p["ms"] <= 0:
+        return None, None
+    return p[key] / (p["ms"] * 1e-3), p["ms"] / max(p["launches"], 1)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +123,7 @@ def main() -> None:
     ap.add_argument("--ts", type=int, default=960)
     ap.add_argument("--cpu-sample", type=int, default=16000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true", help="time the FP64 path only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
@@ -103,33 +141,78 @@ def main() -> None:
     t0 = time.perf_counter()
     m.zvg(theta, e)
     t_zvg = time.perf_counter() - t0
+    print(f"bench: n={n} ts={ts} zvg {t_zvg:.2f}s", file=sys.stderr, flush=True)
 
-    for _ in range(args.warmup):
-        m.loglik(theta)
-    mle.synchronize()
-    mle.profile_enable(True)
-    phases = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ll = m.loglik(theta)
-        phases.append(m.phase_times())
-    mle.synchronize()
-    elapsed = time.perf_counter() - t0
-    upd = mle.profile_get("tile_update")
-    gen = mle.profile_get("gen")
-    trsv = mle.profile_get("trsv")
-    mle.profile_enable(False)
-    status = m.status
-    dotp = m.dotp
+    f64 = _timed(m, theta, args.steps, args.warmup, ["tile_update", "gen", "trsv", "potrf_diag", "trsm_panel"])
+    print(f"bench: fp64 {f64['elapsed'] / args.steps:.3f}s/eval l={f64['ll']:.10f}", file=sys.stderr, flush=True)
+    mix = None
+    if not args.fp64_only:
+        m.precision = 2
+        m.set_mixed(**MIXED)
+        mix = _timed(m, theta, args.steps, args.warmup,
+                     ["tile_update_f64", "tile_update_f32", "tile_update_bf16", "gen", "convert", "trsv", "symv",
+                      "potrf_diag", "trsm_panel"])
+        pm = m.prec_map()
+        low = pm[np.tril_indices(pm.shape[0])]
+        mix["tiles"] = {"f64": int(np.sum(low == 0)), "f32": int(np.sum(low == 1)), "bf16": int(np.sum(low == 3))}
+        print(f"bench: mixed {mix['elapsed'] / args.steps:.3f}s/eval l={mix['ll']:.10f}", file=sys.stderr,
+              flush=True)
     m.close()
 
-    ms_per_step = 1000.0 * elapsed / args.steps
-    value = args.steps / elapsed
     eval_flops = flops_dpotrf(n) + n * (n + 1.0)
-    achieved = upd["flops"] / (upd["ms"] * 1e-3) / 1e12 if upd["ms"] > 0 else 0.0
+    f64_value = args.steps / f64["elapsed"]
     # z = L(theta) e, so at the same theta L^-1 z = e exactly in exact
     # arithmetic: a size-independent check of factor + solve.
     ee = float(e @ e)
+    upd = f64["prof"]["tile_update"]
+    a64, l64 = _rate(upd, "flops")
+    roof64 = {
+        "kernel": "k_tile_update (FP64 MFMA trailing update, 192x192 blocks)",
+        "bound": "mfma", "achieved": a64 / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        "frac": a64 / 1e12 / PEAK_FP64_TFLOPS, "traffic": None, "launches": upd["launches"], "avg_launch_ms": l64,
+    }
+    fp64_rec = {
+        "value": f64_value, "ms_per_step": 1000.0 * f64["elapsed"] / args.steps, "loglik": f64["ll"],
+        "status": f64["status"], "phases_s": f64["phases"],
+        "tflops_effective": eval_flops * f64_value / 1e12,
+        "solve_check_rel": abs(f64["dotp"] - ee) / ee,
+        "roofline": roof64,
+        "gen_GBps": (_rate(f64["prof"]["gen"], "bytes")[0] or 0) / 1e9,
+        "trsv_GBps": (_rate(f64["prof"]["trsv"], "bytes")[0] or 0) / 1e9,
+    }
+    if mix is None:
+        value, ms_per_step, path, dtype = f64_value, fp64_rec["ms_per_step"], "fp64", "f64"
+        roofline = roof64
+        extra = {}
+    else:
+        value = args.steps / mix["elapsed"]
+        ms_per_step = 1000.0 * mix["elapsed"] / args.steps
+        path, dtype = "mixed", "f64/f32/bf16"
+        kernels = {}
+        for name, peak, label in [("tile_update_f64", PEAK_FP64_TFLOPS, "FP64 MFMA"),
+                                  ("tile_update_f32", PEAK_FP32_TFLOPS, "FP32 MFMA"),
+                                  ("tile_update_bf16", PEAK_BF16_TFLOPS, "BF16 MFMA, FP32 accumulation")]:
+            p = mix["prof"][name]
+            a, lm = _rate(p, "flops")
+            if a is None:
+                continue
+            kernels[name] = {"kernel": f"k_update_class ({label})", "bound": "mfma", "achieved": a / 1e12,
+                             "peak": peak, "unit": "TFLOP/s", "frac": a / 1e12 / peak, "traffic": None,
+                             "launches": p["launches"], "avg_launch_ms": lm, "total_ms": p["ms"] / args.steps}
+        dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+        roofline = dict(kernels[dom])
+        extra = {
+            "rel_dl_vs_fp64": abs(mix["ll"] - f64["ll"]) / abs(f64["ll"]),
+            "rel_dlogdet_vs_fp64": abs(mix["logdet"] - f64["logdet"]) / abs(f64["logdet"]),
+            "rel_ddotp_vs_fp64": abs(mix["dotp"] - f64["dotp"]) / abs(f64["dotp"]),
+            "speedup_vs_fp64": value / f64_value,
+            "loglik": mix["ll"], "status": mix["status"], "phases_s": mix["phases"], "tiles": mix["tiles"],
+            "mixed_options": MIXED, "update_kernels": kernels,
+            "symv_GBps": (_rate(mix["prof"]["symv"], "bytes")[0] or 0) / 1e9,
+            "trsv_ms_total": mix["prof"]["trsv"]["ms"] / args.steps,
+            "symv_ms_total": mix["prof"]["symv"]["ms"] / args.steps,
+            "convert_ms": mix["prof"]["convert"]["ms"] / args.steps,
+        }
     rec = {
         "metric": "log-lik evals/s at n=90k, FP64 vs mixed-prec+adjoint on 1 MI355X at stated |Δℓ|",
         "value": value,
@@ -141,38 +224,19 @@ def main() -> None:
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": dtype,
         "data": "synthetic (GenerateXYLoc seed 0, z = L(theta) e with e = dlarnv(3,{0,0,0,1}))",
         "config": {
-            "workload": f"FP64 exact Matern log-likelihood, n={n}, ts={ts}, theta=(1,0.1,0.5), "
-                        "one evaluation = gen + dpotrf + logdet + trsv + dot",
-            "n": n, "ts": ts, "theta": theta.tolist(), "path": "fp64",
+            "workload": f"{path} exact Matern log-likelihood, n={n}, ts={ts}, theta=(1,0.1,0.5), "
+                        "one evaluation = gen + dpotrf + logdet + solve + dot",
+            "n": n, "ts": ts, "theta": theta.tolist(), "path": path,
         },
-        "tflops_effective": eval_flops / (ms_per_step * 1e-3) / 1e12,
-        "phases_s": {k: float(np.mean([p[k] for p in phases])) for k in phases[0]},
-        "loglik": ll,
-        "status": status,
-        "solve_check_rel": abs(dotp - ee) / ee,
+        "roofline": roofline,
+        "fp64": fp64_rec,
         "zvg_s": t_zvg,
-        "roofline": {
-            "kernel": "k_tile_update (FP64 MFMA trailing update, 192x192 blocks)",
-            "bound": "mfma",
-            "achieved": achieved,
-            "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
-            "launches": upd["launches"],
-            "avg_launch_ms": upd["ms"] / max(upd["launches"], 1),
-        },
-        "other_kernels": {
-            "gen_GBps": gen["bytes"] / (gen["ms"] * 1e-3) / 1e9 if gen["ms"] > 0 else None,
-            "gen_ms": gen["ms"] / max(gen["launches"], 1),
-            "trsv_GBps": trsv["bytes"] / (trsv["ms"] * 1e-3) / 1e9 if trsv["ms"] > 0 else None,
-            "trsv_ms": trsv["ms"] / max(trsv["launches"], 1),
-        },
         "cpu_baseline": None,
     }
+    rec.update(extra)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         rec["cpu_baseline"] = cpu_baseline(args.cpu_sample, n, theta, threads)

@@ -106,6 +106,7 @@ _SIGNATURES = {
     "exageostat_gpu_synchronize": (None, []),
     "exageostat_gpu_profile_enable": (None, [ctypes.c_int]),
     "exageostat_gpu_profile_reset": (None, []),
+    "exageostat_mixed_prec_map": (ctypes.c_long, [_P(MLEData), _P(ctypes.c_ubyte), ctypes.c_long]),
     "exageostat_gpu_profile_get": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_long), c_double_p, c_double_p,
                                                   c_double_p]),
     "exageostat_gpu_last_status": (ctypes.c_int, []),

@@ -905,6 +905,13 @@ void exageostat_gpu_synchronize(void) {
   }, 0);
 }
 
+long exageostat_mixed_prec_map(MLE_data* data, unsigned char* out, long cap) {
+  return guarded([&]() -> long {
+    if (!data || !data->gpu) return 0;
+    return mixed_prec_map(*static_cast<Context*>(data->gpu), out, cap);
+  }, -1L);
+}
+
 void exageostat_gpu_profile_enable(int on) { Profiler::get().enable(on != 0); }
 
 void exageostat_gpu_profile_reset(void) { Profiler::get().reset(); }

@@ -74,7 +74,8 @@ __global__ void __launch_bounds__(256) k_potrf_block(double* T, int ts, int j, d
 // X L^T = B for one 64-row slab X (in place of B), L lower with nL diagonal
 // 64-blocks whose inverses are in dinv.  Column sweep: for block jb,
 // X_jb = (B_jb - X_<jb L_jb,<jb^T) inv(L_jb,jb)^T.
-__device__ void trsm_slab(double* X, long ldx, const double* L, long ldl, int nL, const double* dinv,
+template <class TX>
+__device__ void trsm_slab(TX* X, long ldx, const double* L, long ldl, int nL, const double* dinv,
                           double* lds) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -84,9 +85,9 @@ __device__ void trsm_slab(double* X, long ldx, const double* L, long ldl, int nL
   constexpr int LDT = 68;
   for (int jb = 0; jb < nL; ++jb) {
     dx4 acc[C64::FM][C64::FN];
-    double* Xj = X + (long)jb * kNB * ldx;
-    acc_load<C64>(acc, Xj, ldx);
-    gemm_nt_acc<C64>(acc, X, ldx, L + jb * kNB, ldl, jb * kNB, lds, -1.0);
+    TX* Xj = X + (long)jb * kNB * ldx;
+    acc_load<C64, TX>(acc, Xj, ldx);
+    gemm_nt_acc<C64, false, TX, double>(acc, X, ldx, L + jb * kNB, ldl, jb * kNB, lds, -1.0);
     double* Ts = lds;
     double* Is = lds + 64 * LDT;
 #pragma unroll
@@ -123,7 +124,7 @@ __device__ void trsm_slab(double* X, long ldx, const double* L, long ldl, int nL
 #pragma unroll
         for (int jj = 0; jj < C64::FN; ++jj) x[i][jj] = mfma_f64(a[i], b[jj], x[i][jj]);
     }
-    acc_update<C64>(x, Xj, ldx, 0.0);
+    acc_update<C64, TX>(x, Xj, ldx, 0.0);
     // X_jb is read back from global memory by the next column block of this
     // same work-group: the barrier's workgroup-scope release/acquire makes
     // the stores visible inside the CU (no device-scope fence, which would
@@ -139,9 +140,17 @@ __global__ void __launch_bounds__(256) k_trsm_panel(TileView v, int k, const dou
   const int s = blockIdx.x;
   const int m = k + 1 + s / q;
   const int rb = s - (s / q) * q;
-  double* X = tile_f64(v, m, k) + rb * kNB;
   const double* L = tile_f64(v, k, k);
-  trsm_slab(X, v.ts, L, v.ts, q, dinv, lds);
+  const long t = tile_index(v, m, k);
+  // Mixed-precision factors store off-diagonal tiles in FP32; the solve
+  // itself runs in FP64 either way.
+  if (v.prec && v.prec[t] != kF64) {
+    float* X = static_cast<float*>(v.ptr[t]) + rb * kNB;
+    trsm_slab<float>(X, v.ts, L, v.ts, q, dinv, lds);
+  } else {
+    double* X = static_cast<double*>(v.ptr[t]) + rb * kNB;
+    trsm_slab<double>(X, v.ts, L, v.ts, q, dinv, lds);
+  }
 }
 
 // Inside a diagonal tile after block j is factored: blocks (i, j), i > j.
@@ -149,7 +158,7 @@ __global__ void __launch_bounds__(256) k_trsm_intra(double* T, int ts, int j, co
   __shared__ __attribute__((aligned(16))) double lds[2 * 64 * 68];
   const int i = j + 1 + blockIdx.x;
   double* X = T + (long)i * kNB + (long)j * kNB * ts;
-  trsm_slab(X, ts, nullptr, ts, 1, dinv_j, lds);
+  trsm_slab<double>(X, ts, nullptr, ts, 1, dinv_j, lds);
 }
 
 // Inside a diagonal tile: blocks (i, l), j < l <= i: B_il -= B_ij B_lj^T.

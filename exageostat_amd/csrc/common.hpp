@@ -22,8 +22,10 @@ namespace exa {
     }                                                                        \
   } while (0)
 
-// Per-tile storage precision.
-enum TilePrec : uint8_t { kF64 = 0, kF32 = 1, kBF16 = 2 };
+// Per-tile storage / update precision: kF64 (FP64 storage and MFMA), kF32
+// (FP32 storage and MFMA), kF32B (FP32 storage, BF16 MFMA operands with FP32
+// accumulation), kBF16 (BF16 storage, generation output only).
+enum TilePrec : uint8_t { kF64 = 0, kF32 = 1, kBF16 = 2, kF32B = 3 };
 
 // Sub-block size used inside a tile by the diagonal factorisation, the
 // triangular solves and the generation kernel.  Tile sizes must be a

@@ -36,7 +36,7 @@ int kernel_num_params(KernelId k) {
   }
 }
 
-static size_t prec_size(uint8_t p) { return p == kF64 ? 8 : (p == kF32 ? 4 : 2); }
+static size_t prec_size(uint8_t p) { return p == kF64 ? 8 : ((p == kF32 || p == kF32B) ? 4 : 2); }
 
 // ------------------------------------------------------------- TileMatrix --
 TileMatrix::~TileMatrix() { release(); }
@@ -53,7 +53,8 @@ void TileMatrix::release() {
   bytes_ = 0;
 }
 
-void TileMatrix::allocate(int m_, int n_, int ts_, bool lower_, const std::vector<uint8_t>& prec) {
+void TileMatrix::allocate(int m_, int n_, int ts_, bool lower_, const std::vector<uint8_t>& prec,
+                          bool fp64_slots) {
   if (ts_ <= 0 || ts_ % kNB != 0 || ts_ > 1024)
     throw std::invalid_argument("tile size must be a multiple of 64 and at most 1024");
   if (lower_ && m_ != n_) throw std::invalid_argument("lower-packed matrices must be square");
@@ -74,7 +75,7 @@ void TileMatrix::allocate(int m_, int n_, int ts_, bool lower_, const std::vecto
   size_t total = 0;
   for (long t = 0; t < ntiles; ++t) {
     off[t] = total;
-    total += ((size_t)ts * ts * prec_size(h_prec_[t]) + 255) & ~(size_t)255;
+    total += ((size_t)ts * ts * (fp64_slots ? 8 : prec_size(h_prec_[t])) + 255) & ~(size_t)255;
   }
   EXA_HIP(hipMalloc(&pool_, total));
   bytes_ = total;
@@ -84,6 +85,17 @@ void TileMatrix::allocate(int m_, int n_, int ts_, bool lower_, const std::vecto
   EXA_HIP(hipMemcpy(d_ptr_, h_ptr_.data(), sizeof(void*) * ntiles, hipMemcpyHostToDevice));
   EXA_HIP(hipMalloc(&d_prec_, ntiles));
   EXA_HIP(hipMemcpy(d_prec_, h_prec_.data(), ntiles, hipMemcpyHostToDevice));
+  fp64_slots_ = fp64_slots;
+}
+
+void TileMatrix::set_prec(const std::vector<uint8_t>& prec) {
+  if (prec.size() != h_prec_.size()) throw std::invalid_argument("precision map size mismatch");
+  if (!fp64_slots_) {
+    for (size_t t = 0; t < prec.size(); ++t)
+      if (prec_size(prec[t]) > prec_size(h_prec_[t])) throw std::invalid_argument("tile slot too small");
+  }
+  h_prec_ = prec;
+  EXA_HIP(hipMemcpy(d_prec_, h_prec_.data(), h_prec_.size(), hipMemcpyHostToDevice));
 }
 
 TileView TileMatrix::view() const {
@@ -120,7 +132,7 @@ void TileMatrix::to_host(double* A, long ld, hipStream_t s) const {
       const uint8_t p = prec(tm, tn);
       if (p == kF64) {
         EXA_HIP(hipMemcpyAsync(buf.data(), tile_raw(tm, tn), buf.size() * 8, hipMemcpyDeviceToHost, s));
-      } else if (p == kF32) {
+      } else if (p == kF32 || p == kF32B) {
         EXA_HIP(hipMemcpyAsync(fbuf.data(), tile_raw(tm, tn), fbuf.size() * 4, hipMemcpyDeviceToHost, s));
       } else {
         EXA_HIP(hipMemcpyAsync(hbuf.data(), tile_raw(tm, tn), hbuf.size() * 2, hipMemcpyDeviceToHost, s));
@@ -132,7 +144,7 @@ void TileMatrix::to_host(double* A, long ld, hipStream_t s) const {
           const size_t e = i + (size_t)j * ts;
           if (p == kF64) {
             val = buf[e];
-          } else if (p == kF32) {
+          } else if (p == kF32 || p == kF32B) {
             val = fbuf[e];
           } else {
             uint32_t u = (uint32_t)hbuf[e] << 16;
@@ -304,7 +316,41 @@ static double potrf_flops(double n) {
   return n * (((1.0 / 6.0) * n + 0.5) * n + 1.0 / 3.0) + n * (((1.0 / 6.0) * n) * n - 1.0 / 6.0);
 }
 
-int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2, bool sync) {
+ClassLists::~ClassLists() {
+  for (auto p : d_pairs)
+    if (p) (void)hipFree(p);
+}
+
+void ClassLists::build(const TileMatrix& A) {
+  const int mt = A.mt;
+  auto rows = [&](int t) { return (double)std::min<long>(A.ts, (long)A.m - (long)t * A.ts); };
+  std::vector<int> pairs[4];
+  for (int c = 0; c < 4; ++c) {
+    off[c].assign(mt + 2, 0);
+    flops[c].assign(1, 0.0);
+  }
+  for (int n = 0; n < mt; ++n) {
+    for (int c = 0; c < 4; ++c) off[c][n] = (long)pairs[c].size();
+    for (int m = n; m < mt; ++m) {
+      const int c = A.prec(m, n);
+      pairs[c].push_back((m << 16) | n);
+      const double f = (m == n) ? rows(n) * (rows(n) + 1.0) : 2.0 * rows(m) * rows(n);
+      flops[c].push_back(flops[c].back() + f);  // times K at use
+    }
+  }
+  for (int c = 0; c < 4; ++c) {
+    off[c][mt] = off[c][mt + 1] = (long)pairs[c].size();
+    if (d_pairs[c]) (void)hipFree(d_pairs[c]);
+    d_pairs[c] = nullptr;
+    if (!pairs[c].empty()) {
+      EXA_HIP(hipMalloc(&d_pairs[c], sizeof(int) * pairs[c].size()));
+      EXA_HIP(hipMemcpy(d_pairs[c], pairs[c].data(), sizeof(int) * pairs[c].size(), hipMemcpyHostToDevice));
+    }
+  }
+}
+
+int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2, bool sync,
+               const ClassLists* lists) {
   const int mt = A.mt;
   const int ts = A.ts;
   ws.ensure(mt, ts);
@@ -321,7 +367,19 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
       profiled("trsm_panel", below * rk * rk, 0.0, s1, [&] { launch_trsm_panel(v, k, ws.dinv(k), s1); });
   };
   auto update = [&](int k, int lo, int hi, hipStream_t s) {
-    profiled("tile_update", update_flops(A, k, lo, hi), 0.0, s, [&] { launch_tile_update(v, k, lo, hi, s); });
+    if (!lists) {
+      profiled("tile_update", update_flops(A, k, lo, hi), 0.0, s, [&] { launch_tile_update(v, k, lo, hi, s); });
+      return;
+    }
+    static const char* names[4] = {"tile_update_f64", "tile_update_f32", "", "tile_update_bf16"};
+    for (int c : {(int)kF64, (int)kF32, (int)kF32B}) {
+      const long beg = lists->off[c][lo];
+      const long end = lists->off[c][std::min(hi + 1, mt)];
+      if (end <= beg) continue;
+      const double fl = (lists->flops[c][end] - lists->flops[c][beg]) * rows(k);
+      profiled(names[c], fl, 0.0, s,
+               [&] { launch_update_class(c, v, k, lists->d_pairs[c] + beg, end - beg, s); });
+    }
   };
   // s1 carries the panel factorisations (small kernels on the CUs the
   // persistent update leaves free); s2 carries the trailing updates: first
@@ -415,6 +473,8 @@ Context::Context(int N_, int ts_, int nZobs, int nZmiss) : N(N_), ts(ts_) {
 }
 
 Context::~Context() {
+  destroy_mixed(mixed);
+  mixed = nullptr;
   if (d_partial) (void)hipFree(d_partial);
   if (s1) (void)hipStreamDestroy(s1);
   if (s2) (void)hipStreamDestroy(s2);

@@ -34,7 +34,11 @@ class TileMatrix {
   TileMatrix& operator=(const TileMatrix&) = delete;
 
   // prec: per-tile precision map (empty: all FP64), indexed like tile_index.
-  void allocate(int m, int n, int ts, bool lower, const std::vector<uint8_t>& prec = {});
+  // fp64_slots: every tile gets an FP64-sized slot, so set_prec() can later
+  // change any tile's precision without reallocating.
+  void allocate(int m, int n, int ts, bool lower, const std::vector<uint8_t>& prec = {}, bool fp64_slots = false);
+  void set_prec(const std::vector<uint8_t>& prec);
+  const std::vector<uint8_t>& prec_map() const { return h_prec_; }
   void release();
   bool allocated() const { return pool_ != nullptr; }
   TileView view() const;
@@ -53,6 +57,7 @@ class TileMatrix {
  private:
   void* pool_ = nullptr;
   size_t bytes_ = 0;
+  bool fp64_slots_ = false;
   std::vector<void*> h_ptr_;
   std::vector<uint8_t> h_prec_;
   void** d_ptr_ = nullptr;
@@ -114,15 +119,32 @@ class CholWorkspace {
   std::vector<hipEvent_t> ev_;
 };
 
+// Tile pairs (m, n) of a mixed-precision factor grouped by the precision
+// class of tile (m, n): pairs[c] (device, packed m << 16 | n) sorted by
+// column then row, off[c][n] = index of the first pair of column >= n,
+// flops[c] = prefix sums of the pairs' algorithmic update flops per step.
+struct ClassLists {
+  int* d_pairs[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<long> off[4];
+  std::vector<double> flops[4];
+  ~ClassLists();
+  void build(const TileMatrix& A);
+};
+
 // Right-looking tile Cholesky with one-step look-ahead on two streams.
 // Returns 0 or the first failing global column + 1 (the matrix is then
 // not positive definite).  Synchronises s_main at the end.
-int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s_main, hipStream_t s_aux, bool sync = true);
+// With lists, the trailing updates run per precision class (mixed factor).
+int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s_main, hipStream_t s_aux, bool sync = true,
+               const ClassLists* lists = nullptr);
 // z <- L^-1 z (trans=0) or L^-T z (trans=1) with the factor from tile_potrf.
 void tile_trsv(const TileMatrix& L, const CholWorkspace& ws, double* z, int trans, hipStream_t s);
 // sum_i log L_ii (deterministic order).
 double tile_logdet(const TileMatrix& L, double* d_partial, hipStream_t s);
 double device_dot(const double* a, const double* b, long n, double* d_partial, hipStream_t s);
+
+struct MixedState;
+void destroy_mixed(MixedState* m);
 
 // Everything one MLE_data needs on the device.
 class Context {
@@ -142,6 +164,7 @@ class Context {
   DeviceVector Zobs, Zmiss, Zactual, gemv_part;
   DeviceLocations lobs, lmiss;
   PhaseTimes last;
+  MixedState* mixed = nullptr;  // mixed-precision factor, created on first use
 
   double loglik(KernelId kid, const double* theta, int metric, double* logdet_out, double* dotp_out, int* info);
   void ensure_partial(long n);

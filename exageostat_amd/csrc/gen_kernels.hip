@@ -25,7 +25,7 @@ __device__ inline uint16_t f32_to_bf16_rne(float f) {
 __device__ inline void store_prec(void* base, long idx, double v, uint8_t prec) {
   if (prec == kF64) {
     static_cast<double*>(base)[idx] = v;
-  } else if (prec == kF32) {
+  } else if (prec == kF32 || prec == kF32B) {
     static_cast<float*>(base)[idx] = (float)v;
   } else {
     static_cast<uint16_t*>(base)[idx] = f32_to_bf16_rne((float)v);

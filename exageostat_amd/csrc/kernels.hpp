@@ -67,7 +67,22 @@ void launch_dot(const double* z, const double* w, long n, double* partial, hipSt
 // diagonal tiles are lower triangular (zero strictly-upper part); x is
 // padded to nt*ts, part holds nt*mt*ts doubles of scratch.
 void launch_gemv_full(const TileView& v, const double* x, double* y, double* part, hipStream_t s);
+// The first stage of launch_gemv_full alone (partials, no sum).
+void launch_gemv_full_partial(const TileView& v, const double* x, double* part, hipStream_t s);
+// y[i] = sum_{c < ncols} part[c*len + i], i < m.
+void launch_sum_columns(const double* part, int ncols, long len, double* y, long m, hipStream_t s);
 // Zero the strictly upper part of every diagonal tile of a lower matrix.
 void launch_zero_upper_diag(const TileView& v, hipStream_t s);
+
+// ---- mixed precision ---------------------------------------------------
+// dst <- src tile by tile (dst tiles FP64 or FP32 by dst.prec).
+void launch_convert_mixed(const TileView& src, const TileView& dst, long ntiles, hipStream_t s);
+// Trailing update of step k for the tile pairs (packed m << 16 | n) of one
+// precision class (kF64, kF32 or kF32B) of the updated tile.
+void launch_update_class(int cls, const TileView& v, int k, const int* pairs, long npairs, hipStream_t s);
+// out = a*x + b*y (n entries).
+void launch_axpby(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s);
+// Transposed half of y = Sigma x for a lower FP64 matrix (see mixed_kernels).
+void launch_gemv_t_full_partial(const TileView& v, const double* x, double* part, hipStream_t s);
 
 }  // namespace exa

@@ -56,6 +56,22 @@ __device__ __forceinline__ double2 ld_global2(const double* p) {
 #endif
 }
 
+__device__ __forceinline__ float2 ld_global_f2(const float* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef __attribute__((address_space(1))) const float2 gfloat2;
+  return *reinterpret_cast<gfloat2*>(reinterpret_cast<uintptr_t>(p));
+#else
+  return *reinterpret_cast<const float2*>(p);
+#endif
+}
+
+// Two consecutive elements of a tile stored as T, widened to FP64.
+__device__ __forceinline__ double2 ld_pair(const double* p) { return ld_global2(p); }
+__device__ __forceinline__ double2 ld_pair(const float* p) {
+  const float2 f = ld_global_f2(p);
+  return double2{(double)f.x, (double)f.y};
+}
+
 template <class C>
 __device__ __forceinline__ void zero_acc(dx4 (&acc)[C::FM][C::FN]) {
 #pragma unroll
@@ -66,10 +82,12 @@ __device__ __forceinline__ void zero_acc(dx4 (&acc)[C::FM][C::FN]) {
 
 // K must be a multiple of BK; lda/ldb even and A/B 16-byte aligned.
 // B_KMAJOR: B is given as K x BN column-major (B(k, c) at B[k + c*ldb])
-// instead of BN x K.  Ends with a barrier, so the caller may reuse the LDS.
-template <class C, bool B_KMAJOR = false>
-__device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const double* __restrict__ A, long lda,
-                                            const double* __restrict__ B, long ldb, int K, double* lds,
+// instead of BN x K.  TA/TB: storage type of A/B (FP32 tiles of the
+// mixed-precision factor are widened on load).  Ends with a barrier, so the
+// caller may reuse the LDS.
+template <class C, bool B_KMAJOR = false, class TA = double, class TB = double>
+__device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const TA* __restrict__ A, long lda,
+                                            const TB* __restrict__ B, long ldb, int K, double* lds,
                                             double sign) {
   if (K <= 0) return;
   const int tid = threadIdx.x;
@@ -88,7 +106,7 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const doub
       const int e = tid + t * C::NT;
       const int c = e / (C::BM / 2);
       const int r2 = e - c * (C::BM / 2);
-      ra[t] = ld_global2(A + (long)(k0 + c) * lda + 2 * r2);
+      ra[t] = ld_pair(A + (long)(k0 + c) * lda + 2 * r2);
     }
 #pragma unroll
     for (int t = 0; t < C::B_LOADS; ++t) {
@@ -96,11 +114,11 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const doub
       if (B_KMAJOR) {
         const int c = e / (C::BK / 2);
         const int k2 = e - c * (C::BK / 2);
-        rb[t] = ld_global2(B + (long)c * ldb + k0 + 2 * k2);
+        rb[t] = ld_pair(B + (long)c * ldb + k0 + 2 * k2);
       } else {
         const int c = e / (C::BN / 2);
         const int r2 = e - c * (C::BN / 2);
-        rb[t] = ld_global2(B + (long)(k0 + c) * ldb + 2 * r2);
+        rb[t] = ld_pair(B + (long)(k0 + c) * ldb + 2 * r2);
       }
     }
   };
@@ -160,8 +178,8 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const doub
 }
 
 // C[r + c*ldc] = beta*C + acc for the block owned by this work-group.
-template <class C>
-__device__ __forceinline__ void acc_update(const dx4 (&acc)[C::FM][C::FN], double* Cp, long ldc, double beta) {
+template <class C, class T = double>
+__device__ __forceinline__ void acc_update(const dx4 (&acc)[C::FM][C::FN], T* Cp, long ldc, double beta) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm = wave / C::WGN;
@@ -174,14 +192,14 @@ __device__ __forceinline__ void acc_update(const dx4 (&acc)[C::FM][C::FN], doubl
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int r = wm * C::WM + i * 16 + frag_row(lane, v);
-        double* p = Cp + r + (long)c * ldc;
-        *p = (beta == 0.0 ? 0.0 : beta * *p) + acc[i][j][v];
+        T* p = Cp + r + (long)c * ldc;
+        *p = (T)((beta == 0.0 ? 0.0 : beta * (double)*p) + acc[i][j][v]);
       }
     }
 }
 
-template <class C>
-__device__ __forceinline__ void acc_load(dx4 (&acc)[C::FM][C::FN], const double* Cp, long ldc) {
+template <class C, class T = double>
+__device__ __forceinline__ void acc_load(dx4 (&acc)[C::FM][C::FN], const T* Cp, long ldc) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm = wave / C::WGN;
@@ -194,7 +212,7 @@ __device__ __forceinline__ void acc_load(dx4 (&acc)[C::FM][C::FN], const double*
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int r = wm * C::WM + i * 16 + frag_row(lane, v);
-        acc[i][j][v] = Cp[r + (long)c * ldc];
+        acc[i][j][v] = (double)Cp[r + (long)c * ldc];
       }
     }
 }

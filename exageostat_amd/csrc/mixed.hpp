@@ -16,5 +16,8 @@ struct MixedOptions {
 
 double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric, const MixedOptions& opt,
                     double* logdet_out, double* dotp_out, int* info);
+// Copies the tile precision map (TilePrec, lower-packed tile order) of the
+// last mixed evaluation into out[0:cap); returns the tile count (0: none).
+long mixed_prec_map(const Context& ctx, uint8_t* out, long cap);
 
 }  // namespace exa

@@ -1,0 +1,235 @@
+// Kernels of the mixed-precision tile Cholesky: FP64 -> mixed conversion of
+// the generated covariance, trailing updates per precision class of the
+// updated tile (FP64 MFMA, FP32 MFMA, BF16 MFMA with FP32 accumulation) and
+// the transposed half of the symmetric matrix-vector product used by the
+// FP64 iterative refinement.
+//
+// The reference's mixed tier (chameleon_psdpotrf, exageostat_exact/src/
+// compute/psdportf.c:44-187) keeps a band of |m-n| < diag_thick tiles in
+// FP64 and the rest in FP32; here the class of every tile is chosen per theta
+// from the largest correlation the tile can hold, and far tiles are updated
+// with BF16 operands.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "mfma.hpp"
+#include "mfma_lp.hpp"
+
+#include <cstdlib>
+#include <type_traits>
+
+namespace exa {
+
+// ------------------------------------------------------------ conversion --
+// dst (mixed) <- src (FP64, same tiling); 4 work-groups per tile.
+__global__ void __launch_bounds__(256) k_convert_mixed(TileView src, TileView dst) {
+  const long t = blockIdx.x >> 2;
+  const int part = blockIdx.x & 3;
+  const long nel = (long)src.ts * src.ts;
+  const long beg = part * (nel / 4), end = beg + nel / 4;
+  const double* s = static_cast<const double*>(src.ptr[t]);
+  const uint8_t p = dst.prec ? dst.prec[t] : (uint8_t)kF64;
+  if (p == kF64) {
+    double* d = static_cast<double*>(dst.ptr[t]);
+    for (long i = beg + 2 * threadIdx.x; i < end; i += 512)
+      *reinterpret_cast<double2*>(d + i) = *reinterpret_cast<const double2*>(s + i);
+  } else {
+    float* d = static_cast<float*>(dst.ptr[t]);
+    for (long i = beg + 2 * threadIdx.x; i < end; i += 512) {
+      const double2 v = *reinterpret_cast<const double2*>(s + i);
+      *reinterpret_cast<float2*>(d + i) = float2{(float)v.x, (float)v.y};
+    }
+  }
+}
+
+void launch_convert_mixed(const TileView& src, const TileView& dst, long ntiles, hipStream_t s) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_convert_mixed, dim3((unsigned)(ntiles * 4)), dim3(256), 0, s, src, dst);
+  EXA_HIP(hipGetLastError());
+}
+
+// --------------------------------------------------------- class updates --
+// One block of C(m,n) -= A(m,k) B(n,k)^T, C of precision class CLS.
+template <int CLS, int BS>
+struct ClassCfg;
+// FP64 class: the FP64 core.
+template <int BS>
+struct ClassCfg<kF64, BS> {
+  using C = typename std::conditional<BS == 192, GemmCfg<192, 192, 2, 4, 16>,
+            typename std::conditional<BS == 128, GemmCfg<128, 128, 2, 2, 16>, GemmCfg<64, 64, 2, 2, 16>>::type>::type;
+  static constexpr int LDS_BYTES = C::LDS_DOUBLES * 8;
+};
+template <int BS>
+struct ClassCfg<kF32, BS> {
+  using C = typename std::conditional<BS == 192, LpCfg<192, 192, 2, 4, 32>,
+            typename std::conditional<BS == 128, LpCfg<128, 128, 2, 2, 32>, LpCfg<64, 64, 2, 2, 32>>::type>::type;
+  static constexpr int LDS_BYTES = F32Lds<C>::FLOATS * 4;
+};
+template <int BS>
+struct ClassCfg<kF32B, BS> {
+  using C = typename std::conditional<BS == 192, LpCfg<192, 192, 2, 4, 64>,
+            typename std::conditional<BS == 128, LpCfg<128, 128, 2, 2, 64>, LpCfg<64, 64, 2, 2, 64>>::type>::type;
+  static constexpr int LDS_BYTES = Bf16Lds<C>::HALVES * 2;
+};
+
+template <int CLS, int BS, class TA, class TB>
+__device__ __forceinline__ void class_block(const TA* A, const TB* B, void* Cv, int ts, unsigned char* lds) {
+  using Cfg = ClassCfg<CLS, BS>;
+  using C = typename Cfg::C;
+  if constexpr (CLS == kF64) {
+    dx4 acc[C::FM][C::FN];
+    zero_acc<C>(acc);
+    gemm_nt_acc<C, false, TA, TB>(acc, A, ts, B, ts, ts, reinterpret_cast<double*>(lds), -1.0);
+    acc_update<C, double>(acc, static_cast<double*>(Cv), ts, 1.0);
+  } else if constexpr (CLS == kF32) {
+    fx4 acc[C::FM][C::FN];
+    zero_acc_lp<C>(acc);
+    gemm_nt_sub_f32<C, TA, TB>(acc, A, ts, B, ts, ts, reinterpret_cast<float*>(lds));
+    acc_add_lp<C, float>(acc, static_cast<float*>(Cv), ts);
+  } else {
+    fx4 acc[C::FM][C::FN];
+    zero_acc_lp<C>(acc);
+    gemm_nt_sub_bf16<C, TA, TB>(acc, A, ts, B, ts, ts, reinterpret_cast<uint16_t*>(lds));
+    acc_add_lp<C, float>(acc, static_cast<float*>(Cv), ts);
+  }
+}
+
+// Persistent update over the class-CLS tile pairs pairs[0..npairs) (packed
+// m << 16 | n), all blocks of each pair; same XCD-aware split of the block
+// list as the FP64 update.
+template <int CLS, int BS>
+__global__ void __launch_bounds__((ClassCfg<CLS, BS>::C::NT))
+    k_update_class(TileView v, int k, const int* pairs, long nblocks, int nx) {
+  using C = typename ClassCfg<CLS, BS>::C;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[ClassCfg<CLS, BS>::LDS_BYTES];
+  const int q = v.ts / BS;
+  const int w = blockIdx.x;
+  const int x = w % nx;
+  const int gx = gridDim.x / nx;
+  const long per = (nblocks + nx - 1) / nx;
+  const long start = (long)x * per;
+  const long end = start + per < nblocks ? start + per : nblocks;
+  for (long b = start + w / nx; b < end; b += gx) {
+    const long p = b / (q * q);
+    const int rem = (int)(b - p * (q * q));
+    const int bi = rem % q;
+    const int bj = rem / q;
+    const int pk = pairs[p];
+    const int m = pk >> 16;
+    const int n = pk & 0xFFFF;
+    if (m == n && (bi + 1) * C::BM <= bj * C::BN) continue;
+    const long ta = tile_index(v, m, k), tb = tile_index(v, n, k), tc = tile_index(v, m, n);
+    const bool fa = v.prec[ta] != kF64, fb = v.prec[tb] != kF64;
+    const long ao = bi * C::BM, bo = bj * C::BN;
+    const long co = bi * C::BM + (long)bj * C::BN * v.ts;
+    void* Cv = (CLS == kF64) ? (void*)(static_cast<double*>(v.ptr[tc]) + co)
+                             : (void*)(static_cast<float*>(v.ptr[tc]) + co);
+    if (fa) {
+      const float* A = static_cast<const float*>(v.ptr[ta]) + ao;
+      if (fb) class_block<CLS, BS>(A, static_cast<const float*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
+      else class_block<CLS, BS>(A, static_cast<const double*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
+    } else {
+      const double* A = static_cast<const double*>(v.ptr[ta]) + ao;
+      if (fb) class_block<CLS, BS>(A, static_cast<const float*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
+      else class_block<CLS, BS>(A, static_cast<const double*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
+    }
+  }
+}
+
+template <int CLS, int BS>
+static int class_grid_cap() {
+  static int cap = 0;
+  if (cap) return cap;
+  int dev = 0, cus = 0, occ = 1;
+  EXA_HIP(hipGetDevice(&dev));
+  EXA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  EXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_update_class<CLS, BS>,
+                                                       ClassCfg<CLS, BS>::C::NT, 0));
+  int reserve = 16;
+  if (const char* e = getenv("EXAGEOSTAT_RESERVE_CUS")) reserve = atoi(e);
+  if (reserve < 0 || reserve >= cus) reserve = 0;
+  cap = (cus - reserve) * (occ > 0 ? occ : 1);
+  if (cap >= 8) cap -= cap % 8;
+  if (cap < 1) cap = 1;
+  return cap;
+}
+
+template <int CLS, int BS>
+static void launch_class_bs(const TileView& v, int k, const int* pairs, long npairs, hipStream_t s) {
+  const long q = v.ts / BS;
+  const long blocks = npairs * q * q;
+  const int cap = class_grid_cap<CLS, BS>();
+  const int grid = blocks < cap ? (int)blocks : cap;
+  const int nx = (grid % 8 == 0) ? 8 : 1;
+  hipLaunchKernelGGL((k_update_class<CLS, BS>), dim3((unsigned)grid), dim3(ClassCfg<CLS, BS>::C::NT), 0, s, v, k,
+                     pairs, blocks, nx);
+  EXA_HIP(hipGetLastError());
+}
+
+template <int CLS>
+static void launch_class(const TileView& v, int k, const int* pairs, long npairs, hipStream_t s) {
+  if (v.ts % 192 == 0) launch_class_bs<CLS, 192>(v, k, pairs, npairs, s);
+  else if (v.ts % 128 == 0) launch_class_bs<CLS, 128>(v, k, pairs, npairs, s);
+  else launch_class_bs<CLS, 64>(v, k, pairs, npairs, s);
+}
+
+void launch_update_class(int cls, const TileView& v, int k, const int* pairs, long npairs, hipStream_t s) {
+  if (npairs <= 0) return;
+  if (cls == kF64) launch_class<kF64>(v, k, pairs, npairs, s);
+  else if (cls == kF32) launch_class<kF32>(v, k, pairs, npairs, s);
+  else launch_class<kF32B>(v, k, pairs, npairs, s);
+}
+
+// ------------------------------------------------ symmetric product (IR) --
+// part[(mt + m) * mt*ts + n*ts + col] = (A(m,n)^T x_m)[col] for m > n (zero
+// otherwise), for a lower-packed FP64 matrix whose diagonal tiles are full:
+// together with the A(m,n) x_n partials of k_gemv_full_partial this gives
+// y = Sigma x.  One work-group per (tile n, 64 columns, tile m).
+__global__ void __launch_bounds__(256) k_gemv_t_full_partial(TileView v, const double* x, double* part_out) {
+  __shared__ double part[64];
+  const int tid = threadIdx.x;
+  const int r = tid & 63;
+  const int g = tid >> 6;
+  const int ts = v.ts;
+  const int q = ts / kNB;
+  const int n = blockIdx.x / q;
+  const int cb = blockIdx.x - n * q;
+  const int m = blockIdx.y;
+  const long len = (long)v.mt * ts;
+  double* out = part_out + (long)(v.mt + m) * len + (long)n * ts + cb * kNB;
+  if (m <= n) {
+    if (tid < 64) out[r] = 0.0;
+    return;
+  }
+  const double* A = tile_f64(v, m, n);
+  const double* xm = x + (long)m * ts;
+  for (int cc = g * 16; cc < g * 16 + 16; ++cc) {
+    const int col = cb * kNB + cc;
+    double s = 0.0;
+    for (int row = r; row < ts; row += 64) s += A[row + (long)col * ts] * xm[row];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (r == 0) part[cc] = s;
+  }
+  __syncthreads();
+  if (tid < 64) out[r] = part[r];
+}
+
+void launch_gemv_t_full_partial(const TileView& v, const double* x, double* part, hipStream_t s) {
+  const int q = v.ts / kNB;
+  hipLaunchKernelGGL(k_gemv_t_full_partial, dim3((unsigned)(v.nt * q), (unsigned)v.mt), dim3(256), 0, s, v, x,
+                     part);
+  EXA_HIP(hipGetLastError());
+}
+
+__global__ void k_axpby(double a, const double* x, double b, const double* y, double* out, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a * x[i] + b * y[i];
+}
+
+void launch_axpby(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, x, b, y, out, n);
+  EXA_HIP(hipGetLastError());
+}
+
+}  // namespace exa

@@ -19,6 +19,28 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// sum over columns c = g, g+4, ... < ts of A[c*ts] * y[c] (A points at one
+// row of a column-major tile).
+template <class T>
+__device__ inline double row_dot(const T* A, int ts, const double* y, int g) {
+  double s0 = 0.0, s1 = 0.0;
+  int c = g;
+  for (; c + 4 < ts; c += 8) {
+    s0 += (double)A[(long)c * ts] * y[c];
+    s1 += (double)A[(long)(c + 4) * ts] * y[c + 4];
+  }
+  for (; c < ts; c += 4) s0 += (double)A[(long)c * ts] * y[c];
+  return s0 + s1;
+}
+
+// this lane's share (rows r, r+64, ...) of sum_rows col[row] * y[row].
+template <class T>
+__device__ inline double col_dot(const T* col, int ts, const double* y, int r) {
+  double s = 0.0;
+  for (int row = r; row < ts; row += 64) s += (double)col[row] * y[row];
+  return s;
+}
+
 // y = X t (trans = 0) or y = X^T t (trans = 1) for one lower-triangular
 // ts x ts tile X (the inverse of a diagonal factor tile); one work-group
 // per 64 rows (trans = 0) or 64 columns (trans = 1).
@@ -68,15 +90,13 @@ __global__ void __launch_bounds__(256) k_gemv_panel(TileView v, int k, const dou
   const int rb = blockIdx.x - (blockIdx.x / q) * q;
   for (int i = tid; i < ts; i += 256) y[i] = yk[i];
   __syncthreads();
-  const double* A = tile_f64(v, m, k) + rb * kNB + r;
-  double s0 = 0.0, s1 = 0.0;
-  int c = g;
-  for (; c + 4 < ts; c += 8) {
-    s0 += A[(long)c * ts] * y[c];
-    s1 += A[(long)(c + 4) * ts] * y[c + 4];
+  const long ti = tile_index(v, m, k);
+  // Tiles of a mixed-precision factor may be stored in FP32.
+  if (v.prec && v.prec[ti] != kF64) {
+    part[g][r] = row_dot(static_cast<const float*>(v.ptr[ti]) + rb * kNB + r, ts, y, g);
+  } else {
+    part[g][r] = row_dot(static_cast<const double*>(v.ptr[ti]) + rb * kNB + r, ts, y, g);
   }
-  for (; c < ts; c += 4) s0 += A[(long)c * ts] * y[c];
-  part[g][r] = s0 + s1;
   __syncthreads();
   if (tid < 64) {
     z[(long)m * ts + rb * kNB + r] -= (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
@@ -96,11 +116,12 @@ __global__ void __launch_bounds__(256) k_gemv_t_row(TileView v, int k, const dou
   const int cb = blockIdx.x - m * q;
   for (int i = tid; i < ts; i += 256) y[i] = yk[i];
   __syncthreads();
-  const double* A = tile_f64(v, k, m);
+  const long ti = tile_index(v, k, m);
+  const bool f32 = v.prec && v.prec[ti] != kF64;
   for (int cc = g * 16; cc < g * 16 + 16; ++cc) {
     const int col = cb * kNB + cc;
-    double s = 0.0;
-    for (int row = r; row < ts; row += 64) s += A[row + (long)col * ts] * y[row];
+    double s = f32 ? col_dot(static_cast<const float*>(v.ptr[ti]) + (long)col * ts, ts, y, r)
+                   : col_dot(static_cast<const double*>(v.ptr[ti]) + (long)col * ts, ts, y, r);
     s = wave_sum(s);
     if (r == 0) part[cc] = s;
   }
@@ -220,6 +241,18 @@ void launch_dot(const double* z, const double* w, long n, double* partial, hipSt
   const long blocks = (n + 4095) / 4096;
   if (blocks <= 0) return;
   hipLaunchKernelGGL(k_dot, dim3((unsigned)blocks), dim3(256), 0, s, z, w, n, partial);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_gemv_full_partial(const TileView& v, const double* x, double* part, hipStream_t s) {
+  const int q = v.ts / kNB;
+  hipLaunchKernelGGL(k_gemv_full_partial, dim3((unsigned)(v.mt * q), (unsigned)v.nt), dim3(256), 0, s, v, x,
+                     part);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_sum_columns(const double* part, int ncols, long len, double* y, long m, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_columns, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, part, ncols, len, y, m);
   EXA_HIP(hipGetLastError());
 }
 

@@ -151,6 +151,44 @@ class ExactMLE:
         _check(self.lib, "MLE_alg")
         return (val, g) if grad else val
 
+    def set_mixed(self, fp32_corr: float = 0.0, bf16: bool = False, fp64_corr: float = 0.0,
+                  diag_thick: int = 1, ir_steps: int = 3) -> None:
+        """Mixed-precision options (used when precision == 2): tiles whose
+        largest correlation is >= fp64_corr stay FP64, tiles below fp32_corr
+        get BF16 updates when bf16 is on, the rest are FP32; the solve is
+        refined ir_steps times in FP64."""
+        d = self.data
+        d.mp_fp32_corr = fp32_corr
+        d.mp_bf16 = 1 if bf16 else 0
+        d.mp_fp64_corr = fp64_corr
+        d.diag_thick = diag_thick
+        d.mp_ir_steps = ir_steps
+
+    @property
+    def precision(self) -> int:
+        return self.data.precision
+
+    @precision.setter
+    def precision(self, p: int) -> None:
+        self.data.precision = p
+
+    def prec_map(self) -> np.ndarray:
+        """Per-tile precision classes (0 FP64, 1 FP32, 3 BF16 updates) of the
+        last mixed evaluation as an mt x mt lower-triangular array (-1 above
+        the diagonal)."""
+        ntiles = self.lib.exageostat_mixed_prec_map(ctypes.byref(self.data), None, 0)
+        _check(self.lib, "exageostat_mixed_prec_map")
+        buf = (ctypes.c_ubyte * max(ntiles, 1))()
+        self.lib.exageostat_mixed_prec_map(ctypes.byref(self.data), buf, ntiles)
+        mt = (self.N + self.ts - 1) // self.ts
+        out = -np.ones((mt, mt), dtype=np.int8)
+        idx = 0
+        for n in range(mt):
+            for m in range(n, mt):
+                out[m, n] = buf[idx] if ntiles else -1
+                idx += 1
+        return out
+
     @property
     def status(self) -> int:
         return self.data.status

@@ -232,6 +232,10 @@ void exageostat_dlarnv(int idist, int *iseed, int n, double *x);
  * launching stream ("tile_update", "potrf_diag", "trsm_panel", "gen", "trsv"),
  * used by the benchmark; off by default. */
 void exageostat_gpu_synchronize(void);
+/* No counterpart in the reference.  Tile precision classes (0 FP64, 1 FP32,
+ * 3 FP32 storage with BF16 updates) of the last mixed-precision evaluation
+ * (precision = 2), lower-packed column by column; returns the tile count. */
+long exageostat_mixed_prec_map(MLE_data *data, unsigned char *out, long cap);
 void exageostat_gpu_profile_enable(int on);
 void exageostat_gpu_profile_reset(void);
 int exageostat_gpu_profile_get(const char *name, long *launches, double *ms, double *flops, double *bytes);
