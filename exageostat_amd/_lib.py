@@ -101,6 +101,7 @@ _SIGNATURES = {
     "EXAGEOSTAT_Lapack_to_Tile": (ctypes.c_int, [c_double_p, ctypes.c_int, _P(Desc)]),
     "EXAGEOSTAT_Tile_to_Lapack": (ctypes.c_int, [_P(Desc), c_double_p, ctypes.c_int]),
     "EXAGEOSTAT_dpotrf_Tile": (ctypes.c_int, [ctypes.c_int, _P(Desc)]),
+    "EXAGEOSTAT_dpotri_Tile": (ctypes.c_int, [ctypes.c_int, _P(Desc)]),
     "EXAGEOSTAT_dtrsv_Tile": (ctypes.c_int, [ctypes.c_int, _P(Desc), c_double_p]),
     "exageostat_dlarnv": (None, [ctypes.c_int, _P(ctypes.c_int), ctypes.c_int, c_double_p]),
     "exageostat_gpu_synchronize": (None, []),

@@ -1,0 +1,235 @@
+// Kernels of the adjoint gradient: the inverse Z = Sigma^-1 from the tile
+// Cholesky factor (block Takahashi recurrence, FP64 MFMA) and the fused
+// regeneration of dSigma/dtheta reduced against Z - alpha alpha^T.
+//
+// The reference has no gradient: its optimiser is derivative-free (NLOPT
+// BOBYQA over MLE_alg, src/compute/MLE_exact.c / examples/examples.c:
+// 263-264), and its derivative tiles (core_dcmg_matern_dsigma_square,
+// core_dcmg_matern_dbeta, core_dcmg_matern_dnu in exageostat_exact/
+// cpu_core/compute/core_zcmg.c:315-488) are not wired to a gradient.
+//
+// Takahashi recurrence.  With Sigma = L L^T, Z L = L^-T is upper
+// triangular, which for tile column k (from the last to the first) gives
+//   P_jk = L_jk X_kk                       (X_kk = L_kk^-1, j > k)
+//   Z_mk = - sum_{j>k} Z(m,j) P_jk         (m > k; Z(m,j) = Z_jm^T for j > m)
+//   Z_kk = X_kk^T X_kk - sum_{j>k} Z_jk^T P_jk.
+// Z overwrites L in place (lower tiles; diagonal tiles full).
+#include "besselk.hpp"
+#include "common.hpp"
+#include "kernels.hpp"
+#include "mfma.hpp"
+
+namespace exa {
+
+using C192 = GemmCfg<192, 192, 2, 4, 16>;
+using C128 = GemmCfg<128, 128, 2, 2, 16>;
+using CT64 = GemmCfg<64, 64, 2, 2, 16>;
+
+// P_j = L_jk X_kk for j in (k, mt); P holds mt tiles of ts x ts, indexed by j.
+template <class C>
+__global__ void __launch_bounds__(C::NT) k_trmm_panel(TileView v, int k, const double* X, double* P) {
+  __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
+  const int qm = v.ts / C::BM, qn = v.ts / C::BN;
+  const long b = blockIdx.x;
+  const int j = k + 1 + (int)(b / (qm * qn));
+  const int rem = (int)(b % (qm * qn));
+  const int bi = rem % qm, bj = rem / qm;
+  const long ts = v.ts;
+  // X_kk is lower triangular: rows below bj*BN only.
+  const int k0 = bj * C::BN;
+  const double* A = tile_f64(v, j, k) + bi * C::BM + (long)k0 * ts;
+  const double* B = X + k0 + (long)bj * C::BN * ts;  // B(c, kk) = X[k0 + kk, bj*BN + c]
+  dx4 acc[C::FM][C::FN];
+  zero_acc<C>(acc);
+  gemm_nt_acc<C, true>(acc, A, ts, B, ts, (int)(ts - k0), lds, 1.0);
+  acc_update<C>(acc, P + (long)j * ts * ts + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
+}
+
+// Z_mk = - sum_{j>k} Z(m,j) P_j for m in (k, mt).
+template <class C>
+__global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, const double* P) {
+  __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
+  const int qm = v.ts / C::BM, qn = v.ts / C::BN;
+  const long b = blockIdx.x;
+  const int m = k + 1 + (int)(b / (qm * qn));
+  const int rem = (int)(b % (qm * qn));
+  const int bi = rem % qm, bj = rem / qm;
+  const long ts = v.ts;
+  dx4 acc[C::FM][C::FN];
+  zero_acc<C>(acc);
+  for (int j = k + 1; j < v.mt; ++j) {
+    const double* B = P + (long)j * ts * ts + (long)bj * C::BN * ts;  // B(c, kk) = P_j[kk, bj*BN + c]
+    if (j <= m) {
+      const double* A = tile_f64(v, m, j) + bi * C::BM;
+      gemm_nt_acc<C, true>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
+    } else {
+      const double* A = tile_f64(v, j, m) + (long)bi * C::BM * ts;  // A(r, kk) = Z_jm[kk, bi*BM + r]
+      gemm_nt_acc<C, true, double, double, true>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
+    }
+  }
+  acc_update<C>(acc, tile_f64(v, m, k) + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
+}
+
+// Split-k terms of Z_kk: Q_0 = X_kk^T X_kk, Q_i = -Z_{k+i,k}^T P_{k+i}.
+template <class C>
+__global__ void __launch_bounds__(C::NT) k_takahashi_diag(TileView v, int k, const double* X, const double* P,
+                                                          double* Q) {
+  __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
+  const int qm = v.ts / C::BM, qn = v.ts / C::BN;
+  const long b = blockIdx.x;
+  const int i = (int)(b / (qm * qn));
+  const int rem = (int)(b % (qm * qn));
+  const int bi = rem % qm, bj = rem / qm;
+  const long ts = v.ts;
+  dx4 acc[C::FM][C::FN];
+  zero_acc<C>(acc);
+  if (i == 0) {
+    // X^T X: X(kk, r) = 0 for kk < r, so the sum starts at the larger block.
+    const int k0 = (bi * C::BM > bj * C::BN ? bi * C::BM : bj * C::BN);
+    gemm_nt_acc<C, true, double, double, true>(acc, X + k0 + (long)bi * C::BM * ts, ts,
+                                               X + k0 + (long)bj * C::BN * ts, ts, (int)(ts - k0), lds, 1.0);
+  } else {
+    const int j = k + i;
+    gemm_nt_acc<C, true, double, double, true>(acc, tile_f64(v, j, k) + (long)bi * C::BM * ts, ts,
+                                               P + (long)j * ts * ts + (long)bj * C::BN * ts, ts, (int)ts, lds,
+                                               -1.0);
+  }
+  acc_update<C>(acc, Q + (long)i * ts * ts + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
+}
+
+// out[e] = sum_{i < cnt} Q[i*len + e] (fixed order), e < len.
+__global__ void k_sum_tiles(const double* Q, int cnt, long len, double* out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= len) return;
+  double s = 0.0;
+  for (int i = 0; i < cnt; ++i) s += Q[(long)i * len + e];
+  out[e] = s;
+}
+
+template <class C>
+static void takahashi_cfg(const TileView& v, int k, const double* X, double* P, double* Q, int stage,
+                          hipStream_t s) {
+  const long per = (long)(v.ts / C::BM) * (v.ts / C::BN);
+  const long below = v.mt - k - 1;
+  if (stage == 0 && below > 0)
+    hipLaunchKernelGGL(k_trmm_panel<C>, dim3((unsigned)(below * per)), dim3(C::NT), 0, s, v, k, X, P);
+  if (stage == 1 && below > 0)
+    hipLaunchKernelGGL(k_takahashi_col<C>, dim3((unsigned)(below * per)), dim3(C::NT), 0, s, v, k,
+                       (const double*)P);
+  if (stage == 2)
+    hipLaunchKernelGGL(k_takahashi_diag<C>, dim3((unsigned)((below + 1) * per)), dim3(C::NT), 0, s, v, k, X,
+                       (const double*)P, Q);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_takahashi_stage(const TileView& v, int k, const double* X, double* P, double* Q, double* Zkk,
+                            int stage, hipStream_t s) {
+  if (v.ts % 192 == 0) takahashi_cfg<C192>(v, k, X, P, Q, stage, s);
+  else if (v.ts % 128 == 0) takahashi_cfg<C128>(v, k, X, P, Q, stage, s);
+  else takahashi_cfg<CT64>(v, k, X, P, Q, stage, s);
+  if (stage == 2) {
+    const long len = (long)v.ts * v.ts;
+    hipLaunchKernelGGL(k_sum_tiles, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, (const double*)Q,
+                       v.mt - k, len, Zkk);
+    EXA_HIP(hipGetLastError());
+  }
+}
+
+// ------------------------------------------------ dSigma/dtheta reduction --
+__device__ inline double grad_distance(double x1, double y1, double x2, double y2, int metric) {
+  if (metric == 1) {
+    const double d2r = M_PI / 180.0;
+    const double lat1 = x1 * d2r, lon1 = y1 * d2r, lat2 = x2 * d2r, lon2 = y2 * d2r;
+    const double u = sin((lat2 - lat1) * 0.5);
+    const double w = sin((lon2 - lon1) * 0.5);
+    return 2.0 * 6371.0 * asin(sqrt(u * u + cos(lat1) * cos(lat2) * w * w));
+  }
+  const double dx = x2 - x1, dy = y2 - y1;
+  return sqrt(dx * dx + dy * dy);
+}
+
+// One work-group per 64 x 64 block of a lower tile (m, n), 64 x 4 threads:
+// sum over its entries (i >= j on diagonal tiles, off-diagonal entries
+// counted twice) of (Z_ij - alpha_i alpha_j) * dSigma_ij/dtheta_p.
+__global__ void __launch_bounds__(256) k_grad_matern(TileView v, MaternGradParams P, const double* alpha,
+                                                     double* partial) {
+  __shared__ double red[4][4];
+  const int q = v.ts / kNB;
+  const long b = blockIdx.x;
+  const long t = b / (q * q);
+  const int rem = (int)(b - t * (q * q));
+  const int bj = rem / q, bi = rem - (rem / q) * q;
+  int m, n;
+  tri_decode(t, m, n);
+  const int ts = v.ts;
+  const double* Z = tile_f64(v, m, n);
+  const int i = bi * kNB + threadIdx.x;
+  const long gi = (long)m * ts + i;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (gi < P.n) {
+    const double xi = P.x[gi], yi = P.y[gi], ai = alpha[gi];
+    for (int jj = threadIdx.y; jj < kNB; jj += 4) {
+      const int j = bj * kNB + jj;
+      const long gj = (long)n * ts + j;
+      if (gj >= P.n) break;
+      if (m == n && i < j) continue;
+      const double w = ((m == n && i == j) ? 1.0 : 2.0) * (Z[i + (long)j * ts] - ai * alpha[gj]);
+      const double d = grad_distance(xi, yi, P.x[gj], P.y[gj], P.metric);
+      if (d == 0.0) {
+        acc[0] += w;
+        if (P.nugget) acc[3] += w;
+        continue;
+      }
+      const double xx = d / P.beta;
+      Dual kn, kn1;
+      knu_pair(P.ko, xx, kn, kn1);
+      const Dual xnu = exp(P.ko.nu * ::log(xx));  // x^nu with d/dnu = x^nu ln x
+      const Dual corr = P.con * xnu * kn;
+      acc[0] += w * corr.v;
+      acc[1] += w * P.sigma2 * P.con.v * (xnu.v * xx * kn1.v - 2.0 * P.ko.nu.v * xnu.v * kn.v) / P.beta;
+      acc[2] += w * P.sigma2 * corr.d;
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    double s = acc[p];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (threadIdx.x == 0) red[threadIdx.y][p] = s;
+  }
+  __syncthreads();
+  if (threadIdx.y == 0 && threadIdx.x < 4) {
+    const int p = threadIdx.x;
+    partial[b * 4 + p] = (red[0][p] + red[1][p]) + (red[2][p] + red[3][p]);
+  }
+}
+
+// out[p * G + g] = sum over b = g, g + G, ... < nb of partial[b*4 + p].
+__global__ void __launch_bounds__(256) k_reduce_partials4(const double* partial, long nb, double* out) {
+  __shared__ double red[4];
+  const int p = blockIdx.y;
+  const int G = gridDim.x;
+  double s = 0.0;
+  for (long b = (long)blockIdx.x + (long)threadIdx.x * G; b < nb; b += (long)G * 256) s += partial[b * 4 + p];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[p * G + blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+long grad_partial_blocks(const TileView& v) {
+  const long q = v.ts / kNB;
+  return (long)v.mt * (v.mt + 1) / 2 * q * q;
+}
+
+void launch_grad_matern(const TileView& v, const MaternGradParams& P, const double* alpha, double* partial,
+                        double* out, int groups, hipStream_t s) {
+  const long nb = grad_partial_blocks(v);
+  hipLaunchKernelGGL(k_grad_matern, dim3((unsigned)nb), dim3(64, 4), 0, s, v, P, alpha, partial);
+  EXA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_partials4, dim3((unsigned)groups, 4), dim3(256), 0, s, (const double*)partial, nb,
+                     out);
+  EXA_HIP(hipGetLastError());
+}
+
+}  // namespace exa

@@ -399,6 +399,33 @@ int EXAGEOSTAT_dpotrf_Tile(CHAM_enum uplo, CHAM_desc_t* A) {
   }, -1);
 }
 
+int EXAGEOSTAT_dpotri_Tile(CHAM_enum uplo, CHAM_desc_t* A) {
+  return guarded([&]() {
+    if (uplo != ChamLower) throw std::invalid_argument("only ChamLower is supported");
+    auto* impl = static_cast<DescImpl*>(A->mat);
+    TileMatrix& tm = *impl->tm;
+    if (!tm.lower) throw std::invalid_argument("descriptor must be lower-packed");
+    const size_t bytes = sizeof(double) * (size_t)tm.ts * tm.ts * tm.mt;
+    double *P = nullptr, *Q = nullptr;
+    EXA_HIP(hipMalloc(&P, bytes));
+    if (hipMalloc(&Q, bytes) != hipSuccess) {
+      (void)hipFree(P);
+      throw std::runtime_error("potri: out of device memory");
+    }
+    try {
+      tile_potri_inplace(tm, rt().ws, P, Q, rt().s1);
+      EXA_HIP(hipStreamSynchronize(rt().s1));
+    } catch (...) {
+      (void)hipFree(P);
+      (void)hipFree(Q);
+      throw;
+    }
+    (void)hipFree(P);
+    (void)hipFree(Q);
+    return 0;
+  }, -1);
+}
+
 int EXAGEOSTAT_dtrsv_Tile(CHAM_enum trans, CHAM_desc_t* L, double* b) {
   return guarded([&]() {
     auto* impl = static_cast<DescImpl*>(L->mat);

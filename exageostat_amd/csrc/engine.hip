@@ -410,6 +410,7 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
   profiled("trtri_diag", (double)mt * ts * (double)ts * ts / 3.0, 0.0, s1,
            [&] { launch_trtri_tiles(v, ws.dinv_base(), ws.xinv_base(), s1); });
   EXA_HIP(hipMemcpyAsync(ws.host_info(), d_info, sizeof(int), hipMemcpyDeviceToHost, s1));
+  ws.factored = A.tile_raw(0, 0);
   if (!sync) return 0;
   EXA_HIP(hipStreamSynchronize(s1));
   return *ws.host_info();
@@ -474,6 +475,7 @@ Context::Context(int N_, int ts_, int nZobs, int nZmiss) : N(N_), ts(ts_) {
 
 Context::~Context() {
   destroy_mixed(mixed);
+  destroy_adjoint(adjoint);
   mixed = nullptr;
   if (d_partial) (void)hipFree(d_partial);
   if (s1) (void)hipStreamDestroy(s1);

@@ -74,7 +74,7 @@ struct DeviceVector {
 };
 
 struct PhaseTimes {
-  double gen = 0, facto = 0, logdet = 0, solve = 0, total = 0;
+  double gen = 0, facto = 0, logdet = 0, solve = 0, grad = 0, total = 0;
 };
 
 // Device copy of a location set (x, y), cached by host pointer.
@@ -109,6 +109,8 @@ class CholWorkspace {
   int* host_info() const { return h_info_; }  // pinned host mirror
   hipEvent_t ev(int i);
   int mt_ = 0, ts_ = 0;
+  // First tile of the matrix whose factorisation filled dinv/xinv last.
+  const void* factored = nullptr;
 
  private:
   double* dinv_ = nullptr;
@@ -145,6 +147,8 @@ double device_dot(const double* a, const double* b, long n, double* d_partial, h
 
 struct MixedState;
 void destroy_mixed(MixedState* m);
+struct AdjointState;
+void destroy_adjoint(AdjointState* a);
 
 // Everything one MLE_data needs on the device.
 class Context {
@@ -165,6 +169,7 @@ class Context {
   DeviceLocations lobs, lmiss;
   PhaseTimes last;
   MixedState* mixed = nullptr;  // mixed-precision factor, created on first use
+  AdjointState* adjoint = nullptr;  // gradient buffers, created on first use
 
   double loglik(KernelId kid, const double* theta, int metric, double* logdet_out, double* dotp_out, int* info);
   void ensure_partial(long n);

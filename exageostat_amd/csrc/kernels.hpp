@@ -85,4 +85,29 @@ void launch_axpby(double a, const double* x, double b, const double* y, double* 
 // Transposed half of y = Sigma x for a lower FP64 matrix (see mixed_kernels).
 void launch_gemv_t_full_partial(const TileView& v, const double* x, double* part, hipStream_t s);
 
+// ---- adjoint gradient ---------------------------------------------------
+struct MaternGradParams {
+  const double* x;
+  const double* y;
+  long n;       // logical matrix size
+  int metric;   // 0 Euclidean, 1 great circle
+  int nugget;   // 1: d/dtau^2 (fourth component) where the distance is 0
+  double sigma2;
+  double beta;
+  Dual con;               // 1/(2^(nu-1) Gamma(nu)) and its nu-derivative
+  KnuOrder<Dual> ko;      // K_nu constants with nu-derivatives
+};
+// One stage of tile column k of the Takahashi sweep Z = Sigma^-1 over the
+// factor in v (0: P_j = L_jk X_kk; 1: Z_mk for m > k; 2: Z_kk into Zkk,
+// with Q as split-k scratch of mt tiles).  X = L_kk^-1, P: mt tiles.
+void launch_takahashi_stage(const TileView& v, int k, const double* X, double* P, double* Q, double* Zkk,
+                            int stage, hipStream_t s);
+// Number of partial blocks of launch_grad_matern (partial: 4 per block).
+long grad_partial_blocks(const TileView& v);
+// out[p*groups + g], p < 4: partial sums of sum_ij w_ij (Z_ij - a_i a_j)
+// dSigma_ij/dtheta_p over the lower tiles of Z (w = 1 on the diagonal, 2
+// elsewhere); p = sigma^2, beta, nu, nugget.
+void launch_grad_matern(const TileView& v, const MaternGradParams& P, const double* alpha, double* partial,
+                        double* out, int groups, hipStream_t s);
+
 }  // namespace exa

@@ -82,10 +82,10 @@ __device__ __forceinline__ void zero_acc(dx4 (&acc)[C::FM][C::FN]) {
 
 // K must be a multiple of BK; lda/ldb even and A/B 16-byte aligned.
 // B_KMAJOR: B is given as K x BN column-major (B(k, c) at B[k + c*ldb])
-// instead of BN x K.  TA/TB: storage type of A/B (FP32 tiles of the
-// mixed-precision factor are widened on load).  Ends with a barrier, so the
-// caller may reuse the LDS.
-template <class C, bool B_KMAJOR = false, class TA = double, class TB = double>
+// instead of BN x K; A_KMAJOR likewise for A (A(r, k) at A[k + r*lda]).
+// TA/TB: storage type of A/B (FP32 tiles of the mixed-precision factor are
+// widened on load).  Ends with a barrier, so the caller may reuse the LDS.
+template <class C, bool B_KMAJOR = false, class TA = double, class TB = double, bool A_KMAJOR = false>
 __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const TA* __restrict__ A, long lda,
                                             const TB* __restrict__ B, long ldb, int K, double* lds,
                                             double sign) {
@@ -104,9 +104,15 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const TA* 
 #pragma unroll
     for (int t = 0; t < C::A_LOADS; ++t) {
       const int e = tid + t * C::NT;
-      const int c = e / (C::BM / 2);
-      const int r2 = e - c * (C::BM / 2);
-      ra[t] = ld_pair(A + (long)(k0 + c) * lda + 2 * r2);
+      if (A_KMAJOR) {
+        const int r = e / (C::BK / 2);
+        const int k2 = e - r * (C::BK / 2);
+        ra[t] = ld_pair(A + (long)r * lda + k0 + 2 * k2);
+      } else {
+        const int c = e / (C::BM / 2);
+        const int r2 = e - c * (C::BM / 2);
+        ra[t] = ld_pair(A + (long)(k0 + c) * lda + 2 * r2);
+      }
     }
 #pragma unroll
     for (int t = 0; t < C::B_LOADS; ++t) {
@@ -126,9 +132,16 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const TA* 
 #pragma unroll
     for (int t = 0; t < C::A_LOADS; ++t) {
       const int e = tid + t * C::NT;
-      const int c = e / (C::BM / 2);
-      const int r2 = e - c * (C::BM / 2);
-      *reinterpret_cast<double2*>(As + buf * C::A_STAGE + c * C::LDA + 2 * r2) = ra[t];
+      if (A_KMAJOR) {
+        const int r = e / (C::BK / 2);
+        const int k2 = e - r * (C::BK / 2);
+        As[buf * C::A_STAGE + (2 * k2) * C::LDA + r] = ra[t].x;
+        As[buf * C::A_STAGE + (2 * k2 + 1) * C::LDA + r] = ra[t].y;
+      } else {
+        const int c = e / (C::BM / 2);
+        const int r2 = e - c * (C::BM / 2);
+        *reinterpret_cast<double2*>(As + buf * C::A_STAGE + c * C::LDA + 2 * r2) = ra[t];
+      }
     }
 #pragma unroll
     for (int t = 0; t < C::B_LOADS; ++t) {

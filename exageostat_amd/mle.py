@@ -278,6 +278,11 @@ class DeviceMatrix:
         _check(self.lib, "dpotrf")
         return info
 
+    def potri(self) -> None:
+        """Sigma^-1 in place of the factor (call right after potrf)."""
+        self.lib.EXAGEOSTAT_dpotri_Tile(_lib.ChamLower, self.desc)
+        _check(self.lib, "dpotri")
+
     def logdet_half(self) -> float:
         acc = np.zeros(1)
         ud = ctypes.POINTER(Desc)()

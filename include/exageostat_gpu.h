@@ -219,6 +219,11 @@ int EXAGEOSTAT_Lapack_to_Tile(const double *A, int lda, CHAM_desc_t *desc);
 int EXAGEOSTAT_Tile_to_Lapack(CHAM_desc_t *desc, double *A, int lda);
 /* In-place lower Cholesky; returns 0 or the first failing column (1-based). */
 int EXAGEOSTAT_dpotrf_Tile(CHAM_enum uplo, CHAM_desc_t *A);
+/* Sigma^-1 in place of the factor from EXAGEOSTAT_dpotrf_Tile (the last
+ * matrix factored): lower tiles, diagonal tiles written in full.  Replaces
+ * CHAMELEON_dpotri_Tile (a dependency of the reference, not vendored:
+ * chameleon submodule, .gitmodules:4-6), used here by the gradient. */
+int EXAGEOSTAT_dpotri_Tile(CHAM_enum uplo, CHAM_desc_t *A);
 /* B <- L^-1 B (trans = ChamNoTrans) or L^-T B (ChamTrans), B an n-vector
  * given in host memory; L from EXAGEOSTAT_dpotrf_Tile. */
 int EXAGEOSTAT_dtrsv_Tile(CHAM_enum trans, CHAM_desc_t *L, double *b);

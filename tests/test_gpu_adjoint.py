@@ -1,0 +1,118 @@
+"""GPU adjoint gradient: Sigma^-1 from the tile factor (potri) and
+d l / d theta through MLE_alg's grad argument.
+
+Checks: potri against numpy's inverse; the gradient against the CPU oracle's
+dense gradient (itself pinned to finite differences on the CPU); against
+central finite differences of the GPU FP64 l (h = 1e-6 relative) at a size
+the oracle does not reach.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import datagen, likelihood, matern
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+K = "univariate_matern_stationary"
+KN = "univariate_matern_nuggets_stationary"
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import exageostat_amd
+    exageostat_amd.load()
+    assert exageostat_amd.device_count() >= 1, "no HIP device visible"
+    return exageostat_amd
+
+
+@pytest.mark.parametrize("n,ts", [(700, 64), (1600, 320), (2500, 960), (1000, 192), (1100, 128)])
+def test_potri_matches_inverse(lib, n, ts):
+    """Lower tiles (and full diagonal tiles) of Sigma^-1 within 1e-12 of
+    numpy's inverse, relative to max |Sigma^-1|; padded last tile."""
+    x, y, _ = datagen.generate_xy_loc(n, 5)
+    theta = (1.0, 0.1, 0.5)
+    S = matern.core_dcmg(x, y, x, y, theta) + 0.05 * np.eye(n)
+    M = lib.DeviceMatrix(n, n, ts, lower=True)
+    M.from_numpy(S)
+    assert M.potrf() == 0
+    M.potri()
+    Z = M.to_numpy()
+    R = np.linalg.inv(S)
+    mt = (n + ts - 1) // ts
+    mask = np.zeros((n, n), dtype=bool)
+    for tn in range(mt):
+        mask[tn * ts:, tn * ts:(tn + 1) * ts] = True
+    err = np.max(np.abs(Z[mask] - R[mask])) / np.max(np.abs(R))
+    assert err <= 1e-12, err
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return dict(np.load(os.path.join(GOLD, "c1_seed0.npz")))
+
+
+@pytest.mark.parametrize("theta", [(1.0, 0.1, 0.5), (1.3, 0.07, 0.8), (0.7, 0.15, 1.37), (1.0, 0.1, 1.5)])
+def test_gradient_matches_oracle(lib, c1, theta):
+    """n = 1600 (ts = 320): every component within 1e-8 of the oracle's
+    dense gradient, relative to the gradient's largest component."""
+    l_ref, g_ref = likelihood.gradient(K, c1["x"], c1["y"], c1["z"], theta)
+    m = lib.ExactMLE(c1["x"], c1["y"], 320)
+    m.set_z(c1["z"])
+    val, g = m.loglik(theta, grad=True)
+    assert m.status == 0
+    assert abs(val - l_ref) <= 1e-9 * abs(l_ref)
+    scale = np.max(np.abs(g_ref))
+    assert np.max(np.abs(g - g_ref)) <= 1e-8 * scale, (g, g_ref)
+    # the context keeps working after the in-place inverse
+    val2 = m.loglik(theta)
+    assert abs(val2 - l_ref) <= 1e-9 * abs(l_ref)
+    m.close()
+
+
+def test_gradient_nugget_matches_oracle(lib):
+    g = np.load(os.path.join(GOLD, "nugget_900.npz"))
+    theta = np.array(g["theta"], dtype=float)
+    l_ref, g_ref = likelihood.gradient(KN, g["x"], g["y"], g["z"], theta)
+    m = lib.ExactMLE(g["x"], g["y"], 64, kernel=KN)
+    m.set_z(g["z"])
+    val, gr = m.loglik(theta, grad=True)
+    assert len(gr) == 4
+    assert abs(val - l_ref) <= 1e-9 * abs(l_ref)
+    assert np.max(np.abs(gr - g_ref)) <= 1e-8 * np.max(np.abs(g_ref)), (gr, g_ref)
+    m.close()
+
+
+def test_gradient_central_differences_large(lib):
+    """n = 12000 (ts = 960, partial last tile): the adjoint gradient against
+    central differences of the GPU FP64 l with h = 1e-6 * theta_p (the FD
+    truncation and rounding error is ~1e-6 of the gradient's scale here)."""
+    n, ts = 12000, 960
+    theta = np.array([1.0, 0.1, 0.5])
+    x, y, _ = datagen.generate_xy_loc(n, 2)
+    e, _ = datagen.dlarnv(3, [0, 0, 0, 1], n)
+    m = lib.ExactMLE(x, y, ts)
+    m.zvg(theta, e)
+    th = np.array([0.9, 0.12, 0.6])
+    _, g = m.loglik(th, grad=True)
+    fd = np.zeros(3)
+    for p in range(3):
+        h = 1e-6 * th[p]
+        tp, tm = th.copy(), th.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd[p] = (m.loglik(tp) - m.loglik(tm)) / (2 * h)
+    m.close()
+    assert np.max(np.abs(g - fd)) <= 1e-5 * np.max(np.abs(fd)), (g, fd)
+
+
+def test_gradient_non_spd(lib, c1):
+    m = lib.ExactMLE(c1["x"], c1["y"], 320)
+    m.set_z(c1["z"])
+    val, g = m.loglik([-1.0, 0.1, 0.5], grad=True)
+    assert val == -np.inf
+    assert m.status > 0
+    assert np.all(np.isnan(g))
+    m.close()
