@@ -124,7 +124,19 @@ def main() -> None:
     ap.add_argument("--cpu-sample", type=int, default=16000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="time the FP64 path only")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy kernel of BYTES (scales the HBM counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nel = args.copy_calibration // 8
+        a = torch.ones(nel, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": nel * 8, "bytes_written": nel * 8}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
               "(see DESIGN.md for how the path would shard)", file=sys.stderr)

@@ -30,8 +30,10 @@ struct AdjointState {
   long tiles = 0;
   double* partial = nullptr;
   long partial_blocks = 0;
-  double* sums = nullptr;  // 4 * kGroups
+  double* sums = nullptr;  // 8 * kGroups
+  KnuOrder<Dual>* ko = nullptr;  // device: K_nu constants of the bivariate kernel
   ~AdjointState() {
+    if (ko) (void)hipFree(ko);
     if (P) (void)hipFree(P);
     if (Q) (void)hipFree(Q);
     if (partial) (void)hipFree(partial);
@@ -67,8 +69,7 @@ void tile_potri_inplace(TileMatrix& A, const CholWorkspace& ws, double* P, doubl
 
 double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric, double* grad,
                            double* logdet_out, double* dotp_out, int* info) {
-  if (kid != kUnivariate && kid != kNugget)
-    throw std::invalid_argument("the adjoint gradient covers the univariate Matern kernels (with or without nugget)");
+  if (kid != kUnivariate && kid != kNugget && kid != kBivariate) throw std::invalid_argument("unsupported kernel");
   const int np = kernel_num_params(kid);
   const double t0 = wall_seconds();
   int inf = 0;
@@ -99,10 +100,10 @@ double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int 
   if (st.partial_blocks < nb) {
     if (st.partial) (void)hipFree(st.partial);
     st.partial = nullptr;
-    EXA_HIP(hipMalloc(&st.partial, sizeof(double) * 4 * nb));
+    EXA_HIP(hipMalloc(&st.partial, sizeof(double) * 8 * nb));
     st.partial_blocks = nb;
   }
-  if (!st.sums) EXA_HIP(hipMalloc(&st.sums, sizeof(double) * 4 * kGroups));
+  if (!st.sums) EXA_HIP(hipMalloc(&st.sums, sizeof(double) * 8 * kGroups));
   hipStream_t s = ctx.s1;
 
   // alpha = L^-T (L^-1 z); ctx.work holds L^-1 z after loglik.
@@ -112,25 +113,55 @@ double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int 
            [&] { tile_trsv(A, ctx.ws, st.alpha.d, 1, s); });
   tile_potri_inplace(A, ctx.ws, st.P, st.Q, s);
 
-  MaternGradParams G;
-  G.x = ctx.l1.dx;
-  G.y = ctx.l1.dy;
-  G.n = ctx.N;
-  G.metric = metric;
-  G.nugget = kid == kNugget ? 1 : 0;
-  G.sigma2 = theta[0];
-  G.beta = theta[1];
-  G.con = host_matern_con(theta[2]);
-  knu_prepare(theta[2], G.ko);
-  profiled("grad_reduce", 0.0, 4.0 * (double)ctx.N * (ctx.N + 1.0), s,
-           [&] { launch_grad_matern(v, G, st.alpha.d, st.partial, st.sums, kGroups, s); });
-  std::vector<double> h(4 * kGroups);
+  const int nsum = kid == kBivariate ? 6 : 4;
+  if (kid == kBivariate) {
+    // theta = (s1, s2, beta, nu1, nu2, rho); rho~ = rho sqrt(nu1 nu2)/nu12
+    // (the gamma ratios of the parsimonious constants, core_zcmg.c:1614-1655).
+    BivariateGradParams B;
+    B.x = ctx.l1.dx;
+    B.y = ctx.l1.dy;
+    B.n = ctx.N;
+    B.metric = metric;
+    B.s1 = theta[0];
+    B.s2 = theta[1];
+    B.beta = theta[2];
+    B.nu1 = theta[3];
+    B.nu2 = theta[4];
+    B.nu12 = 0.5 * (theta[3] + theta[4]);
+    B.rho_ratio = std::sqrt(B.nu1 * B.nu2) / B.nu12 * std::sqrt(B.s1 * B.s2);
+    B.s12 = theta[5] * B.rho_ratio;
+    B.con1 = host_matern_con(B.nu1);
+    B.con2 = host_matern_con(B.nu2);
+    B.con12 = host_matern_con(B.nu12);
+    KnuOrder<Dual> hko[3];
+    knu_prepare(B.nu1, hko[0]);
+    knu_prepare(B.nu2, hko[1]);
+    knu_prepare(B.nu12, hko[2]);
+    if (!st.ko) EXA_HIP(hipMalloc(&st.ko, sizeof(hko)));
+    EXA_HIP(hipMemcpyAsync(st.ko, hko, sizeof(hko), hipMemcpyHostToDevice, s));
+    profiled("grad_reduce", 0.0, 4.0 * (double)ctx.N * (ctx.N + 1.0), s,
+             [&] { launch_grad_bivariate(v, B, st.ko, st.alpha.d, st.partial, st.sums, kGroups, s); });
+  } else {
+    MaternGradParams G;
+    G.x = ctx.l1.dx;
+    G.y = ctx.l1.dy;
+    G.n = ctx.N;
+    G.metric = metric;
+    G.nugget = kid == kNugget ? 1 : 0;
+    G.sigma2 = theta[0];
+    G.beta = theta[1];
+    G.con = host_matern_con(theta[2]);
+    knu_prepare(theta[2], G.ko);
+    profiled("grad_reduce", 0.0, 4.0 * (double)ctx.N * (ctx.N + 1.0), s,
+             [&] { launch_grad_matern(v, G, st.alpha.d, st.partial, st.sums, kGroups, s); });
+  }
+  std::vector<double> h((size_t)nsum * kGroups);
   EXA_HIP(hipMemcpyAsync(h.data(), st.sums, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
   EXA_HIP(hipStreamSynchronize(s));
   if (Profiler::get().enabled()) Profiler::get().collect();
-  double acc[4] = {0, 0, 0, 0};
-  for (int p = 0; p < 4; ++p)
-    for (int g = 0; g < kGroups; ++g) acc[p] += h[p * kGroups + g];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int p = 0; p < nsum; ++p)
+    for (int g = 0; g < kGroups; ++g) acc[p] += h[(size_t)p * kGroups + g];
   for (int p = 0; p < np; ++p) grad[p] = -0.5 * acc[p];
   const double t1 = wall_seconds();
   ctx.last = lt;

@@ -204,13 +204,93 @@ __global__ void __launch_bounds__(256) k_grad_matern(TileView v, MaternGradParam
   }
 }
 
-// out[p * G + g] = sum over b = g, g + G, ... < nb of partial[b*4 + p].
-__global__ void __launch_bounds__(256) k_reduce_partials4(const double* partial, long nb, double* out) {
+// Correlation M = c x^nu K_nu(x) at x = d/beta with dM/dbeta and dM/dnu.
+__device__ inline void matern_parts(const KnuOrder<Dual>& ko, Dual con, double x, double beta, double& M,
+                                    double& dbeta, double& dnu) {
+  Dual kn, kn1;
+  knu_pair(ko, x, kn, kn1);
+  const Dual xnu = exp(ko.nu * ::log(x));
+  const Dual corr = con * xnu * kn;
+  M = corr.v;
+  dnu = corr.d;
+  dbeta = con.v * xnu.v * (x * kn1.v - 2.0 * ko.nu.v * kn.v) / beta;
+}
+
+// Parsimonious bivariate kernel, rows/columns interleaved by variable
+// (row 2p+a is variable a at location p).  Each thread takes 2 x 2 blocks
+// (one location pair); sums over the lower entries as in k_grad_matern, for
+// theta = (s1, s2, beta, nu1, nu2, rho); partial holds 8 per block.
+__global__ void __launch_bounds__(256) k_grad_bivariate(TileView v, BivariateGradParams P,
+                                                        const KnuOrder<Dual>* ko, const double* alpha,
+                                                        double* partial) {
+  __shared__ double red[4][6];
+  const int q = v.ts / kNB;
+  const long b = blockIdx.x;
+  const long t = b / (q * q);
+  const int rem = (int)(b - t * (q * q));
+  const int bj = rem / q, bi = rem - (rem / q) * q;
+  int m, n;
+  tri_decode(t, m, n);
+  const int ts = v.ts;
+  const double* Z = tile_f64(v, m, n);
+  const int rp = threadIdx.x & 31;
+  const int i = bi * kNB + 2 * rp;
+  const long gi = (long)m * ts + i;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (gi < P.n) {
+    const double xi = P.x[gi >> 1], yi = P.y[gi >> 1];
+    const double a0 = alpha[gi], a1 = alpha[gi + 1];
+    for (int cp = (threadIdx.x >> 5) + 2 * threadIdx.y; cp < kNB / 2; cp += 8) {
+      const int j = bj * kNB + 2 * cp;
+      const long gj = (long)n * ts + j;
+      if (gj >= P.n) break;
+      if (m == n && i < j) continue;
+      const double b0 = alpha[gj], b1 = alpha[gj + 1];
+      // weights of entries (0,0), (1,0), (0,1), (1,1) of the 2 x 2 block
+      const bool diag = (m == n && i == j);
+      const double w00 = (diag ? 1.0 : 2.0) * (Z[i + (long)j * ts] - a0 * b0);
+      const double w10 = 2.0 * (Z[(i + 1) + (long)j * ts] - a1 * b0);
+      const double w01 = diag ? 0.0 : 2.0 * (Z[i + (long)(j + 1) * ts] - a0 * b1);
+      const double w11 = (diag ? 1.0 : 2.0) * (Z[(i + 1) + (long)(j + 1) * ts] - a1 * b1);
+      const double wc = w10 + w01;  // both cross entries carry C12
+      const double d = grad_distance(xi, yi, P.x[gj >> 1], P.y[gj >> 1], P.metric);
+      double M1 = 1.0, M2 = 1.0, M12 = 1.0, db1 = 0.0, db2 = 0.0, db12 = 0.0, dn1 = 0.0, dn2 = 0.0, dn12 = 0.0;
+      if (d != 0.0) {
+        const double xx = d / P.beta;
+        matern_parts(ko[0], P.con1, xx, P.beta, M1, db1, dn1);
+        matern_parts(ko[1], P.con2, xx, P.beta, M2, db2, dn2);
+        matern_parts(ko[2], P.con12, xx, P.beta, M12, db12, dn12);
+      }
+      const double c12 = P.s12 * M12;
+      acc[0] += w00 * M1 + wc * c12 / (2.0 * P.s1);
+      acc[1] += w11 * M2 + wc * c12 / (2.0 * P.s2);
+      acc[2] += w00 * P.s1 * db1 + wc * P.s12 * db12 + w11 * P.s2 * db2;
+      acc[3] += w00 * P.s1 * dn1 + wc * (c12 * (0.5 / P.nu1 - 0.5 / P.nu12) + 0.5 * P.s12 * dn12);
+      acc[4] += w11 * P.s2 * dn2 + wc * (c12 * (0.5 / P.nu2 - 0.5 / P.nu12) + 0.5 * P.s12 * dn12);
+      acc[5] += wc * P.rho_ratio * M12;
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 6; ++p) {
+    double s = acc[p];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (threadIdx.x == 0) red[threadIdx.y][p] = s;
+  }
+  __syncthreads();
+  if (threadIdx.y == 0 && threadIdx.x < 8) {
+    const int p = threadIdx.x;
+    partial[b * 8 + p] = p < 6 ? (red[0][p] + red[1][p]) + (red[2][p] + red[3][p]) : 0.0;
+  }
+}
+
+// out[p * G + g] = sum over b = g, g + G, ... < nb of partial[b*stride + p].
+__global__ void __launch_bounds__(256) k_reduce_partials(const double* partial, long nb, int stride, double* out) {
   __shared__ double red[4];
   const int p = blockIdx.y;
   const int G = gridDim.x;
   double s = 0.0;
-  for (long b = (long)blockIdx.x + (long)threadIdx.x * G; b < nb; b += (long)G * 256) s += partial[b * 4 + p];
+  for (long b = (long)blockIdx.x + (long)threadIdx.x * G; b < nb; b += (long)G * 256) s += partial[b * stride + p];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
@@ -227,7 +307,17 @@ void launch_grad_matern(const TileView& v, const MaternGradParams& P, const doub
   const long nb = grad_partial_blocks(v);
   hipLaunchKernelGGL(k_grad_matern, dim3((unsigned)nb), dim3(64, 4), 0, s, v, P, alpha, partial);
   EXA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_reduce_partials4, dim3((unsigned)groups, 4), dim3(256), 0, s, (const double*)partial, nb,
+  hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)groups, 4), dim3(256), 0, s, (const double*)partial, nb, 4,
+                     out);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_grad_bivariate(const TileView& v, const BivariateGradParams& P, const KnuOrder<Dual>* ko,
+                           const double* alpha, double* partial, double* out, int groups, hipStream_t s) {
+  const long nb = grad_partial_blocks(v);
+  hipLaunchKernelGGL(k_grad_bivariate, dim3((unsigned)nb), dim3(64, 4), 0, s, v, P, ko, alpha, partial);
+  EXA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)groups, 6), dim3(256), 0, s, (const double*)partial, nb, 8,
                      out);
   EXA_HIP(hipGetLastError());
 }

@@ -102,12 +102,28 @@ struct MaternGradParams {
 // with Q as split-k scratch of mt tiles).  X = L_kk^-1, P: mt tiles.
 void launch_takahashi_stage(const TileView& v, int k, const double* X, double* P, double* Q, double* Zkk,
                             int stage, hipStream_t s);
-// Number of partial blocks of launch_grad_matern (partial: 4 per block).
+struct BivariateGradParams {
+  const double* x;
+  const double* y;
+  long n;  // rows (2 x locations)
+  int metric;
+  double s1, s2, beta, nu1, nu2, nu12;
+  double s12;        // rho~ sqrt(s1 s2), the cross variance
+  double rho_ratio;  // d s12 / d rho = sqrt(nu1 nu2)/nu12 sqrt(s1 s2)
+  Dual con1, con2, con12;
+};
+// Number of partial blocks of launch_grad_matern / launch_grad_bivariate
+// (partial: 4, resp. 8, doubles per block).
 long grad_partial_blocks(const TileView& v);
 // out[p*groups + g], p < 4: partial sums of sum_ij w_ij (Z_ij - a_i a_j)
 // dSigma_ij/dtheta_p over the lower tiles of Z (w = 1 on the diagonal, 2
 // elsewhere); p = sigma^2, beta, nu, nugget.
 void launch_grad_matern(const TileView& v, const MaternGradParams& P, const double* alpha, double* partial,
                         double* out, int groups, hipStream_t s);
+// Same for the parsimonious bivariate kernel: out[p*groups + g], p < 6, for
+// theta = (s1, s2, beta, nu1, nu2, rho); ko: device array of the K_nu
+// constants for nu1, nu2, nu12.
+void launch_grad_bivariate(const TileView& v, const BivariateGradParams& P, const KnuOrder<Dual>* ko,
+                           const double* alpha, double* partial, double* out, int groups, hipStream_t s);
 
 }  // namespace exa

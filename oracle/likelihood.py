@@ -81,14 +81,17 @@ def predict(kernel: str, xobs, yobs, zobs, xmiss, ymiss, theta, zactual=None, me
 
 def gradient(kernel: str, x, y, z, theta, metric: int = 0):
     """d l / d theta = 1/2 alpha^T dS alpha - 1/2 tr(S^-1 dS), alpha = S^-1 z,
-    for the univariate Matern kernels (3 parameters, or 4 with the nugget).
-    Returns (l, grad)."""
+    for the univariate Matern kernels (3 parameters, or 4 with the nugget)
+    and the parsimonious bivariate kernel (6 parameters).  Returns (l, grad)."""
     nugget = kernel == "univariate_matern_nuggets_stationary"
     S = sigma(kernel, x, y, theta, metric)
     cf = sla.cho_factor(S, lower=True)
     alpha = sla.cho_solve(cf, np.asarray(z, dtype=np.float64))
     Sinv = sla.cho_solve(cf, np.eye(S.shape[0]))
-    dS = matern.matern_derivatives(x, y, x, y, theta, metric, nugget=nugget)
+    if kernel == "bivariate_matern_parsimonious":
+        dS = matern.bivariate_derivatives(x, y, x, y, theta, metric)
+    else:
+        dS = matern.matern_derivatives(x, y, x, y, theta, metric, nugget=nugget)
     g = np.array([0.5 * alpha @ D @ alpha - 0.5 * float(np.sum(Sinv * D)) for D in dS])
     l, _, _ = loglik_from_sigma(S, z)
     return l, g

@@ -114,6 +114,59 @@ def core_dcmg_bivariate_parsimonious(x1, y1, x2, y2, theta, metric: int = 0) -> 
     return out
 
 
+def _matern_parts(expr: np.ndarray, nu: float, beta: float):
+    """For expr = d/beta: the correlation M (1 at d = 0) and its derivatives
+    dM/dbeta and dM/dnu (both 0 at d = 0), as in matern_derivatives."""
+    con = 1.0 / (math.pow(2.0, nu - 1.0) * math.gamma(nu))
+    nz = expr != 0
+    e = expr[nz]
+    knu = sp.kv(nu, e)
+    xnu = np.power(e, nu)
+    M = np.ones_like(expr)
+    M[nz] = con * xnu * knu
+    dbeta = np.zeros_like(expr)
+    dbeta[nz] = con * (e * xnu * sp.kv(nu + 1, e) - 2.0 * nu * xnu * knu) / beta
+    dnu = np.zeros_like(expr)
+    dnu[nz] = con * xnu * ((-math.log(2.0) - sp.digamma(nu) + np.log(e)) * knu + dkv_dnu(nu, e))
+    return M, dbeta, dnu
+
+
+def bivariate_derivatives(x1, y1, x2, y2, theta, metric: int = 0):
+    """Partial derivatives of core_dcmg_bivariate_parsimonious with respect to
+    theta = (sigma1^2, sigma2^2, beta, nu1, nu2, rho), as six interleaved
+    (2*m1) x (2*m2) matrices.  The reference has no derivative of this
+    kernel; with rho~ = rho*sqrt(nu1 nu2)/nu12 (the gamma ratios of
+    parsimonious_constants reduce to this) the cross block is
+    C12 = rho~ sqrt(s1 s2) M(d; nu12), so
+      dC12/ds_a  = C12 / (2 s_a),
+      dC12/dnu_a = C12 (1/(2 nu_a) - 1/(2 nu12)) + rho~ sqrt(s1 s2) dM/dnu / 2,
+      dC12/drho  = C12 / rho (written as sqrt(nu1 nu2)/nu12 sqrt(s1 s2) M)."""
+    s1, s2, beta, nu1, nu2, rho = (float(t) for t in theta[:6])
+    nu12, rho_t, s12 = parsimonious_constants(theta)
+    expr = distance_matrix(x1, y1, x2, y2, metric) / beta
+    m1, m2 = expr.shape
+    M1, b1, n1 = _matern_parts(expr, nu1, beta)
+    M2, b2, n2 = _matern_parts(expr, nu2, beta)
+    M12, b12, n12 = _matern_parts(expr, nu12, beta)
+    out = [np.zeros((2 * m1, 2 * m2)) for _ in range(6)]
+
+    def put(mat, a11, a12, a22):
+        mat[0::2, 0::2] = a11
+        mat[1::2, 0::2] = a12
+        mat[0::2, 1::2] = a12
+        mat[1::2, 1::2] = a22
+
+    z = np.zeros_like(expr)
+    c12 = s12 * M12
+    put(out[0], M1, c12 / (2.0 * s1), z)
+    put(out[1], z, c12 / (2.0 * s2), M2)
+    put(out[2], s1 * b1, s12 * b12, s2 * b2)
+    put(out[3], s1 * n1, c12 * (0.5 / nu1 - 0.5 / nu12) + 0.5 * s12 * n12, z)
+    put(out[4], z, c12 * (0.5 / nu2 - 0.5 / nu12) + 0.5 * s12 * n12, s2 * n2)
+    put(out[5], z, math.sqrt(nu1 * nu2) / nu12 * math.sqrt(s1 * s2) * M12, z)
+    return out
+
+
 KERNELS = {
     "univariate_matern_stationary": (3, core_dcmg),
     "univariate_matern_nuggets_stationary": (4, core_dcmg_nuggets),

@@ -85,6 +85,23 @@ def test_gradient_nugget_matches_oracle(lib):
     m.close()
 
 
+def test_gradient_bivariate_matches_oracle(lib):
+    """Parsimonious bivariate kernel, all six parameters (n = 800 rows,
+    ts = 64), against the oracle's dense gradient (pinned to finite
+    differences of l on the CPU)."""
+    g = np.load(os.path.join(GOLD, "bivariate_400.npz"))
+    KB = "bivariate_matern_parsimonious"
+    for theta in (np.array(g["theta"], dtype=float), np.array([0.8, 1.2, 0.12, 0.7, 1.3, -0.4])):
+        l_ref, g_ref = likelihood.gradient(KB, g["x"], g["y"], g["z"], theta)
+        m = lib.ExactMLE(g["x"], g["y"], 64, kernel=KB)
+        m.set_z(g["z"])
+        val, gr = m.loglik(theta, grad=True)
+        m.close()
+        assert len(gr) == 6
+        assert abs(val - l_ref) <= 1e-9 * abs(l_ref)
+        assert np.max(np.abs(gr - g_ref)) <= 1e-8 * np.max(np.abs(g_ref)), (theta, gr, g_ref)
+
+
 def test_gradient_central_differences_large(lib):
     """n = 12000 (ts = 960, partial last tile): the adjoint gradient against
     central differences of the GPU FP64 l with h = 1e-6 * theta_p (the FD

@@ -97,11 +97,13 @@ def test_pick_random_points_split(c1):
     ("univariate_matern_stationary", (1.0, 0.1, 0.5)),
     ("univariate_matern_stationary", (1.3, 0.07, 1.37)),
     ("univariate_matern_nuggets_stationary", (1.0, 0.1, 0.5, 0.01)),
+    ("bivariate_matern_parsimonious", (1.0, 1.3, 0.1, 0.5, 1.0, 0.5)),
 ])
 def test_gradient_matches_central_differences(kernel, theta):
     """The dense adjoint formula vs central differences of dmle, h = 1e-6
     relative (the bar the GPU adjoint is held to as well)."""
-    x, y, _ = datagen.generate_xy_loc(400, 1)
+    nloc = 200 if kernel.startswith("bivariate") else 400
+    x, y, _ = datagen.generate_xy_loc(nloc, 1)
     e, _ = datagen.dlarnv(3, [1, 1, 1, 1], 400)
     z = likelihood.dzvg(kernel, x, y, theta, e)
     _, g = likelihood.gradient(kernel, x, y, z, theta)

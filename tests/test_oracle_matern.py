@@ -93,6 +93,23 @@ def test_derivative_kernels_match_finite_differences(theta):
         assert np.max(np.abs(dS[p] - fd)) < 2e-7 * max(1.0, np.max(np.abs(fd)))
 
 
+@pytest.mark.parametrize("theta", [(1.0, 1.3, 0.1, 0.5, 1.0, 0.5), (0.8, 1.1, 0.2, 1.37, 0.6, -0.3)])
+def test_bivariate_derivatives_match_finite_differences(theta):
+    """Each of the six analytic derivative matrices of the parsimonious
+    bivariate kernel against central differences of the kernel itself."""
+    x, y, _ = datagen.generate_xy_loc(60, 5)
+    dS = matern.bivariate_derivatives(x, y, x, y, theta)
+    base = np.array(theta)
+    for p in range(6):
+        h = 1e-6 * max(1.0, abs(base[p]))
+        tp, tm = base.copy(), base.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd = (matern.core_dcmg_bivariate_parsimonious(x, y, x, y, tp)
+              - matern.core_dcmg_bivariate_parsimonious(x, y, x, y, tm)) / (2 * h)
+        assert np.max(np.abs(dS[p] - fd)) < 2e-7 * max(1.0, np.max(np.abs(fd))), p
+
+
 def test_dkv_dnu_against_mpmath_table():
     g = np.load(os.path.join(GOLD, "knu_table.npz"))
     sel = g["x"] >= 1e-3
