@@ -95,6 +95,9 @@ void TileMatrix::set_prec(const std::vector<uint8_t>& prec) {
       if (prec_size(prec[t]) > prec_size(h_prec_[t])) throw std::invalid_argument("tile slot too small");
   }
   h_prec_ = prec;
+  // The table is read by kernels of earlier launches on the non-blocking
+  // streams; the null-stream copy below does not wait for them.
+  EXA_HIP(hipDeviceSynchronize());
   EXA_HIP(hipMemcpy(d_prec_, h_prec_.data(), h_prec_.size(), hipMemcpyHostToDevice));
 }
 
@@ -184,7 +187,11 @@ void DeviceVector::allocate(long n_, long padded_) {
   n = n_;
   padded = padded_;
   EXA_HIP(hipMalloc(&d, sizeof(double) * padded));
+  // hipMemset runs on the null stream, which does not order with the
+  // library's non-blocking streams: wait for it, or a copy queued next on
+  // another stream can land before the zeros.
   EXA_HIP(hipMemset(d, 0, sizeof(double) * padded));
+  EXA_HIP(hipDeviceSynchronize());
 }
 
 void DeviceVector::release() {

@@ -224,3 +224,35 @@ def test_predict_interpolates(lib, c1):
     zm, _ = m.predict(c1["theta_true"], c1["x"], c1["y"], c1["z"], c1["x"][idx], c1["y"][idx])
     assert np.max(np.abs(zm - c1["z"][idx])) <= 1e-8
     m.close()
+
+
+def test_predict_full_size_via_likelihood(lib):
+    """Kriging at n = 90,000 (1,000 held-out points by the reference's split,
+    89,000 observed), checked without a dense CPU solve: for one held-out
+    point, l(z_obs, z*) over the 89,001 locations is quadratic in z* with its
+    maximum at the kriging mean, so three l evaluations (same Sigma, hence the
+    same factor) recover the mean independently of the prediction path.  The
+    MSE is the reference's mean squared error over the held-out points."""
+    n, ts, theta = 90000, 960, (1.0, 0.1, 0.5)
+    x, y, rng = datagen.generate_xy_loc(n, 0)
+    e, _ = datagen.dlarnv(3, [0, 0, 0, 1], n)
+    m = lib.ExactMLE(x, y, ts)
+    m.zvg(theta, e)
+    z = m.get_z()
+    m.close()
+    xo, yo, zo, xm, ym, za = datagen.pick_random_points(x, y, z, 1000, rng)
+    m = lib.ExactMLE(xo, yo, ts)
+    zm, mse = m.predict(theta, xo, yo, zo, xm, ym, za)
+    m.close()
+    assert abs(mse - np.mean((za - zm) ** 2)) <= 1e-12 * mse
+    for k in (0, 617):
+        m2 = lib.ExactMLE(np.append(xo, xm[k]), np.append(yo, ym[k]), ts)
+        f = []
+        for t in (-1.0, 0.0, 1.0):
+            m2.set_z(np.append(zo, zm[k] + t))
+            f.append(m2.loglik(theta))
+        m2.close()
+        s2 = -1.0 / (f[2] + f[0] - 2.0 * f[1])
+        mu = zm[k] + 0.5 * (f[2] - f[0]) * s2
+        assert 0.0 < s2 < 1.0
+        assert abs(mu - zm[k]) <= 1e-7 * max(1.0, abs(zm[k])), (k, mu, zm[k], s2)
