@@ -107,6 +107,19 @@ def _timed(m, theta, steps: int, warmup: int, names):
     }
 
 
+def _measured_traffic(kernel: str, n: int, ts: int):
+    """HBM bytes per launch of a kernel at this workload, as measured with
+    rocprofv3 FETCH_SIZE/WRITE_SIZE (scaled by a plain copy) and recorded in
+    profiles/hbm_traffic.json; None when no measurement exists."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(f"{kernel}|n={n}|ts={ts}")
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else rec["bytes_per_launch"] / 1e9
+
+
 def _rate(p, key):
     """(achieved units/s, per-launch ms) of one profiled kernel group."""
     if p["ms"] <= 0:
@@ -181,7 +194,8 @@ def main() -> None:
     roof64 = {
         "kernel": "k_tile_update (FP64 MFMA trailing update, 192x192 blocks)",
         "bound": "mfma", "achieved": a64 / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-        "frac": a64 / 1e12 / PEAK_FP64_TFLOPS, "traffic": None, "launches": upd["launches"], "avg_launch_ms": l64,
+        "frac": a64 / 1e12 / PEAK_FP64_TFLOPS, "traffic": _measured_traffic("k_tile_update", n, ts),
+        "traffic_unit": "GB/launch", "launches": upd["launches"], "avg_launch_ms": l64,
     }
     fp64_rec = {
         "value": f64_value, "ms_per_step": 1000.0 * f64["elapsed"] / args.steps, "loglik": f64["ll"],
@@ -209,7 +223,8 @@ def main() -> None:
             if a is None:
                 continue
             kernels[name] = {"kernel": f"k_update_class ({label})", "bound": "mfma", "achieved": a / 1e12,
-                             "peak": peak, "unit": "TFLOP/s", "frac": a / 1e12 / peak, "traffic": None,
+                             "peak": peak, "unit": "TFLOP/s", "frac": a / 1e12 / peak,
+                             "traffic": _measured_traffic(name, n, ts), "traffic_unit": "GB/launch",
                              "launches": p["launches"], "avg_launch_ms": lm, "total_ms": p["ms"] / args.steps}
         dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
         roofline = dict(kernels[dom])
