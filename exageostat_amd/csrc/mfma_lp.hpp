@@ -1,12 +1,13 @@
-// Lower-precision MFMA work-group GEMM cores for the mixed-precision tile
-// Cholesky on CDNA4 (gfx950): FP32 (v_mfma_f32_16x16x4_f32) and BF16 with
-// FP32 accumulation (v_mfma_f32_16x16x32_bf16).
+// Lower-precision MFMA work-group GEMM core for the mixed-precision tile
+// Cholesky on CDNA4 (gfx950): BF16 operands (single or split in two BF16
+// terms) with FP32 accumulation (v_mfma_f32_16x16x32_bf16).
 //
-// Both compute acc -= A * B^T for a BM x BN block, A: BM x K and B: BN x K
+// It computes acc -= A * B^T for a BM x BN block, A: BM x K and B: BN x K
 // column-major tiles stored in FP64 or FP32 (narrowed on load), staged
 // through LDS with double buffering like the FP64 core in mfma.hpp.
-// Result layout of both instructions (measured on gfx950 with an in-bounds
-// single-wave probe): lane l, element v holds D[4*(l/16)+v][l%16].
+// Result layout (measured on gfx950 with an in-bounds single-wave probe):
+// lane l, element v holds D[4*(l/16)+v][l%16]; operand layout: lane l holds
+// A[l%16][8*(l/16)+e] and B^T[l%16][8*(l/16)+e], e = 0..7.
 #pragma once
 
 #include "common.hpp"
@@ -33,12 +34,6 @@ __device__ __forceinline__ float4 ld_quad(const double* p) {
   return float4{(float)a.x, (float)a.y, (float)b.x, (float)b.y};
 }
 
-__device__ __forceinline__ uint16_t bf16_bits_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFF) ? 0x40 : 0));
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
 
 template <int BM_, int BN_, int WGM_, int WGN_, int BK_>
 struct LpCfg {
@@ -79,106 +74,54 @@ __device__ __forceinline__ void acc_add_lp(const fx4 (&acc)[C::FM][C::FN], T* Cp
     }
 }
 
-// ---------------------------------------------------------------- FP32 --
-// LDS holds float slices [k][m] (row stride BM + 4); one slice = BK columns.
-template <class C>
-struct F32Lds {
-  static constexpr int LDA = C::BM + 4, LDB = C::BN + 4;
-  static constexpr int A_STAGE = C::BK * LDA, B_STAGE = C::BK * LDB;
-  static constexpr int FLOATS = 2 * (A_STAGE + B_STAGE);
-  static constexpr int A_LOADS = C::BM * C::BK / 4 / C::NT;
-  static constexpr int B_LOADS = C::BN * C::BK / 4 / C::NT;
-  static_assert(A_LOADS * C::NT * 4 == C::BM * C::BK, "A slice must split evenly");
-  static_assert(B_LOADS * C::NT * 4 == C::BN * C::BK, "B slice must split evenly");
+// ------------------------------------------------------- split BF16 --
+// acc -= A * B^T on the BF16 matrix cores (v_mfma_f32_16x16x32_bf16, FP32
+// accumulation).  NSPLIT = 1: operands rounded to BF16.  NSPLIT = 3: each
+// operand is split as x = hi + lo (hi = bf16(x), lo = bf16(x - hi)) and the
+// product is hi*hi + hi*lo + lo*hi, which keeps about 16 significant bits of
+// each operand (products of BF16 values are exact in FP32) at one third of
+// the BF16 rate, several times the FP32 MFMA rate.
+//
+// LDS holds, per stage, [A_hi | B_hi | A_lo | B_lo] slices [row][k] with
+// row stride BK + 8 (a lane's eight consecutive k values of one row are one
+// 16-byte read; the 16-byte pad keeps the eight rows of a read phase on
+// distinct banks).  Global loads come in units of 4 rows x 8 k (eight
+// 16-byte column pieces), narrowed and split in registers; A units go to
+// the first threads of the work-group, B units to the next ones.
+template <class C, int NSPLIT>
+struct Bf16sLds {
+  static_assert(NSPLIT == 1 || NSPLIT == 3, "one or three BF16 products");
+  static_assert(C::BK % 32 == 0, "BK must be a multiple of the MFMA depth");
+  static constexpr int LDK = C::BK + 8;
+  static constexpr int A_ARR = C::BM * LDK, B_ARR = C::BN * LDK;  // bf16 elements per array
+  static constexpr int STAGE = (NSPLIT == 3 ? 2 : 1) * (A_ARR + B_ARR);
+  static constexpr int HALVES = 2 * STAGE;
+  static constexpr int A_UNITS = (C::BM / 4) * (C::BK / 8);
+  static constexpr int B_UNITS = (C::BN / 4) * (C::BK / 8);
+  static constexpr int SLOTS = (A_UNITS + B_UNITS + C::NT - 1) / C::NT;
 };
 
-template <class C, class TA, class TB>
-__device__ __forceinline__ void gemm_nt_sub_f32(fx4 (&acc)[C::FM][C::FN], const TA* __restrict__ A, long lda,
-                                                const TB* __restrict__ B, long ldb, int K, float* lds) {
-  using L = F32Lds<C>;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm = wave / C::WGN;
-  const int wn = wave - wm * C::WGN;
-  float* As = lds;
-  float* Bs = lds + 2 * L::A_STAGE;
-  float4 ra[L::A_LOADS], rb[L::B_LOADS];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int t = 0; t < L::A_LOADS; ++t) {
-      const int e = tid + t * C::NT;
-      const int c = e / (C::BM / 4), r4 = e - c * (C::BM / 4);
-      ra[t] = ld_quad(A + (long)(k0 + c) * lda + 4 * r4);
-    }
-#pragma unroll
-    for (int t = 0; t < L::B_LOADS; ++t) {
-      const int e = tid + t * C::NT;
-      const int c = e / (C::BN / 4), r4 = e - c * (C::BN / 4);
-      rb[t] = ld_quad(B + (long)(k0 + c) * ldb + 4 * r4);
-    }
-  };
-  auto sstore = [&](int buf) {
-#pragma unroll
-    for (int t = 0; t < L::A_LOADS; ++t) {
-      const int e = tid + t * C::NT;
-      const int c = e / (C::BM / 4), r4 = e - c * (C::BM / 4);
-      *reinterpret_cast<float4*>(As + buf * L::A_STAGE + c * L::LDA + 4 * r4) = ra[t];
-    }
-#pragma unroll
-    for (int t = 0; t < L::B_LOADS; ++t) {
-      const int e = tid + t * C::NT;
-      const int c = e / (C::BN / 4), r4 = e - c * (C::BN / 4);
-      *reinterpret_cast<float4*>(Bs + buf * L::B_STAGE + c * L::LDB + 4 * r4) = rb[t];
-    }
-  };
-  const int nk = K / C::BK;
-  gload(0);
-  sstore(0);
-  __syncthreads();
-  for (int kc = 0; kc < nk; ++kc) {
-    const int buf = kc & 1;
-    if (kc + 1 < nk) gload((kc + 1) * C::BK);
-    const float* Ab = As + buf * L::A_STAGE + wm * C::WM + (lane & 15);
-    const float* Bb = Bs + buf * L::B_STAGE + wn * C::WN + (lane & 15);
-#pragma unroll
-    for (int kk = 0; kk < C::BK; kk += 4) {
-      const int krow = kk + (lane >> 4);
-      float a[C::FM], b[C::FN];
-#pragma unroll
-      for (int i = 0; i < C::FM; ++i) a[i] = -Ab[krow * L::LDA + i * 16];
-#pragma unroll
-      for (int j = 0; j < C::FN; ++j) b[j] = Bb[krow * L::LDB + j * 16];
-#pragma unroll
-      for (int i = 0; i < C::FM; ++i)
-#pragma unroll
-        for (int j = 0; j < C::FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    if (kc + 1 < nk) sstore(buf ^ 1);
-    __syncthreads();
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef __bf16 b2v __attribute__((ext_vector_type(2)));
+
+// Two FP32 values rounded to BF16 (hardware round-to-nearest-even, one
+// packed conversion) and, for NSPLIT = 3, their BF16 remainders.
+template <int NSPLIT>
+__device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
+  const f2v v = {a, b};
+  const b2v h = __builtin_convertvector(v, b2v);
+  hi = __builtin_bit_cast(uint32_t, h);
+  if (NSPLIT == 3) {
+    const f2v back = __builtin_convertvector(h, f2v);
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(v - back, b2v));
   }
 }
 
-// ---------------------------------------------------------------- BF16 --
-// LDS holds bf16 slices [m][k] (row stride BK + 8), so a lane's eight
-// consecutive k values of one row are one 16-byte read.  Global loads are
-// units of 4 rows x 8 k (eight 4-row column pieces), narrowed to bf16 and
-// written as four 16-byte rows; A is negated on the way.
-template <class C>
-struct Bf16Lds {
-  static constexpr int LDK = C::BK + 8;
-  static constexpr int A_STAGE = C::BM * LDK, B_STAGE = C::BN * LDK;  // in bf16 elements
-  static constexpr int HALVES = 2 * (A_STAGE + B_STAGE);
-  static constexpr int A_UNITS = (C::BM / 4) * (C::BK / 8);
-  static constexpr int B_UNITS = (C::BN / 4) * (C::BK / 8);
-  static constexpr int A_SLOTS = (A_UNITS + C::NT - 1) / C::NT;
-  static constexpr int B_SLOTS = (B_UNITS + C::NT - 1) / C::NT;
-};
-
-template <class T>
-__device__ __forceinline__ void load_unit_bf16(const T* base, long ld, uint4 (&rows)[4], bool negate) {
+// Loads one 4-row x 8-k unit (column-major source, leading dimension ld)
+// as BF16 rows (hi) and, for NSPLIT = 3, the BF16 remainders (lo);
+// negate flips the sign bits of both (exact).
+template <int NSPLIT, class T>
+__device__ __forceinline__ void load_unit_split(const T* base, long ld, bool negate, uint4 (&hi)[4], uint4 (&lo)[4]) {
   float v[4][8];
 #pragma unroll
   for (int kk = 0; kk < 8; ++kk) {
@@ -188,68 +131,71 @@ __device__ __forceinline__ void load_unit_bf16(const T* base, long ld, uint4 (&r
     v[2][kk] = q.z;
     v[3][kk] = q.w;
   }
+  const uint32_t sign = negate ? 0x80008000u : 0u;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    uint32_t w[4];
+    uint32_t wh[4], wl[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
-      const float lo = negate ? -v[r][2 * h] : v[r][2 * h];
-      const float hi = negate ? -v[r][2 * h + 1] : v[r][2 * h + 1];
-      w[h] = (uint32_t)bf16_bits_rne(lo) | ((uint32_t)bf16_bits_rne(hi) << 16);
+      split_pair<NSPLIT>(v[r][2 * h], v[r][2 * h + 1], wh[h], wl[h]);
+      wh[h] ^= sign;
+      wl[h] ^= sign;
     }
-    rows[r] = uint4{w[0], w[1], w[2], w[3]};
+    hi[r] = uint4{wh[0], wh[1], wh[2], wh[3]};
+    if (NSPLIT == 3) lo[r] = uint4{wl[0], wl[1], wl[2], wl[3]};
   }
 }
 
-template <class C, class TA, class TB>
-__device__ __forceinline__ void gemm_nt_sub_bf16(fx4 (&acc)[C::FM][C::FN], const TA* __restrict__ A, long lda,
-                                                 const TB* __restrict__ B, long ldb, int K, uint16_t* lds) {
-  using L = Bf16Lds<C>;
+template <class C, int NSPLIT, class TA, class TB>
+__device__ __forceinline__ void gemm_nt_sub_bf16s(fx4 (&acc)[C::FM][C::FN], const TA* __restrict__ A, long lda,
+                                                  const TB* __restrict__ B, long ldb, int K, uint16_t* lds) {
+  using L = Bf16sLds<C, NSPLIT>;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave / C::WGN;
   const int wn = wave - wm * C::WGN;
-  uint16_t* As = lds;
-  uint16_t* Bs = lds + 2 * L::A_STAGE;
-  uint4 ra[L::A_SLOTS][4], rb[L::B_SLOTS][4];
+  uint4 rh[L::SLOTS][4], rl[L::SLOTS][4];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int t = 0; t < L::A_SLOTS; ++t) {
-      const int e = tid + t * C::NT;
-      if (e < L::A_UNITS) {
-        const int mq = e % (C::BM / 4), kg = e / (C::BM / 4);
-        load_unit_bf16(A + (long)(k0 + 8 * kg) * lda + 4 * mq, lda, ra[t], true);
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < L::B_SLOTS; ++t) {
-      const int e = tid + t * C::NT;
-      if (e < L::B_UNITS) {
-        const int mq = e % (C::BN / 4), kg = e / (C::BN / 4);
-        load_unit_bf16(B + (long)(k0 + 8 * kg) * ldb + 4 * mq, ldb, rb[t], false);
+    for (int t = 0; t < L::SLOTS; ++t) {
+      const int u = tid + t * C::NT;
+      if (u < L::A_UNITS) {
+        const int mq = u % (C::BM / 4), kg = u / (C::BM / 4);
+        load_unit_split<NSPLIT>(A + (long)(k0 + 8 * kg) * lda + 4 * mq, lda, true, rh[t], rl[t]);
+      } else if (u < L::A_UNITS + L::B_UNITS) {
+        const int w = u - L::A_UNITS;
+        const int mq = w % (C::BN / 4), kg = w / (C::BN / 4);
+        load_unit_split<NSPLIT>(B + (long)(k0 + 8 * kg) * ldb + 4 * mq, ldb, false, rh[t], rl[t]);
       }
     }
   };
   auto sstore = [&](int buf) {
+    uint16_t* st = lds + buf * L::STAGE;
 #pragma unroll
-    for (int t = 0; t < L::A_SLOTS; ++t) {
-      const int e = tid + t * C::NT;
-      if (e < L::A_UNITS) {
-        const int mq = e % (C::BM / 4), kg = e / (C::BM / 4);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          *reinterpret_cast<uint4*>(As + buf * L::A_STAGE + (4 * mq + r) * L::LDK + 8 * kg) = ra[t][r];
+    for (int t = 0; t < L::SLOTS; ++t) {
+      const int u = tid + t * C::NT;
+      int row, kg;
+      uint16_t* hi_arr;
+      uint16_t* lo_arr;
+      if (u < L::A_UNITS) {
+        row = 4 * (u % (C::BM / 4));
+        kg = u / (C::BM / 4);
+        hi_arr = st;
+        lo_arr = st + L::A_ARR + L::B_ARR;
+      } else if (u < L::A_UNITS + L::B_UNITS) {
+        const int w = u - L::A_UNITS;
+        row = 4 * (w % (C::BN / 4));
+        kg = w / (C::BN / 4);
+        hi_arr = st + L::A_ARR;
+        lo_arr = st + 2 * L::A_ARR + L::B_ARR;
+      } else {
+        continue;
       }
-    }
 #pragma unroll
-    for (int t = 0; t < L::B_SLOTS; ++t) {
-      const int e = tid + t * C::NT;
-      if (e < L::B_UNITS) {
-        const int mq = e % (C::BN / 4), kg = e / (C::BN / 4);
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          *reinterpret_cast<uint4*>(Bs + buf * L::B_STAGE + (4 * mq + r) * L::LDK + 8 * kg) = rb[t][r];
+      for (int r = 0; r < 4; ++r) {
+        *reinterpret_cast<uint4*>(hi_arr + (row + r) * L::LDK + 8 * kg) = rh[t][r];
+        if (NSPLIT == 3) *reinterpret_cast<uint4*>(lo_arr + (row + r) * L::LDK + 8 * kg) = rl[t][r];
       }
     }
   };
@@ -260,20 +206,39 @@ __device__ __forceinline__ void gemm_nt_sub_bf16(fx4 (&acc)[C::FM][C::FN], const
   for (int kc = 0; kc < nk; ++kc) {
     const int buf = kc & 1;
     if (kc + 1 < nk) gload((kc + 1) * C::BK);
-    const uint16_t* Ab = As + buf * L::A_STAGE + (wm * C::WM + (lane & 15)) * L::LDK + 8 * (lane >> 4);
-    const uint16_t* Bb = Bs + buf * L::B_STAGE + (wn * C::WN + (lane & 15)) * L::LDK + 8 * (lane >> 4);
+    const uint16_t* st = lds + buf * L::STAGE;
+    const int aoff = (wm * C::WM + (lane & 15)) * L::LDK + 8 * (lane >> 4);
+    const int boff = (wn * C::WN + (lane & 15)) * L::LDK + 8 * (lane >> 4);
 #pragma unroll
     for (int kk = 0; kk < C::BK; kk += 32) {
-      bf16x8 a[C::FM], b[C::FN];
+      bf16x8 ah[C::FM], bh[C::FN];
 #pragma unroll
-      for (int i = 0; i < C::FM; ++i) a[i] = *reinterpret_cast<const bf16x8*>(Ab + i * 16 * L::LDK + kk);
+      for (int i = 0; i < C::FM; ++i) ah[i] = *reinterpret_cast<const bf16x8*>(st + aoff + i * 16 * L::LDK + kk);
 #pragma unroll
-      for (int j = 0; j < C::FN; ++j) b[j] = *reinterpret_cast<const bf16x8*>(Bb + j * 16 * L::LDK + kk);
+      for (int j = 0; j < C::FN; ++j)
+        bh[j] = *reinterpret_cast<const bf16x8*>(st + L::A_ARR + boff + j * 16 * L::LDK + kk);
+      if (NSPLIT == 3) {
+        bf16x8 al[C::FM], bl[C::FN];
+#pragma unroll
+        for (int i = 0; i < C::FM; ++i)
+          al[i] = *reinterpret_cast<const bf16x8*>(st + L::A_ARR + L::B_ARR + aoff + i * 16 * L::LDK + kk);
+#pragma unroll
+        for (int j = 0; j < C::FN; ++j)
+          bl[j] = *reinterpret_cast<const bf16x8*>(st + 2 * L::A_ARR + L::B_ARR + boff + j * 16 * L::LDK + kk);
+        // small cross terms first, then the leading product
+#pragma unroll
+        for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+          for (int j = 0; j < C::FN; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          }
+      }
 #pragma unroll
       for (int i = 0; i < C::FM; ++i)
 #pragma unroll
         for (int j = 0; j < C::FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
     }
     __builtin_amdgcn_sched_barrier(0);
     if (kc + 1 < nk) sstore(buf ^ 1);

@@ -1,6 +1,7 @@
 // Kernels of the mixed-precision tile Cholesky: FP64 -> mixed conversion of
 // the generated covariance, trailing updates per precision class of the
-// updated tile (FP64 MFMA, FP32 MFMA, BF16 MFMA with FP32 accumulation) and
+// updated tile (FP64 MFMA; split-BF16 MFMA for FP32 tiles; single-BF16 MFMA
+// for far tiles; FP32 accumulation) and
 // the transposed half of the symmetric matrix-vector product used by the
 // FP64 iterative refinement.
 //
@@ -58,17 +59,21 @@ struct ClassCfg<kF64, BS> {
             typename std::conditional<BS == 128, GemmCfg<128, 128, 2, 2, 16>, GemmCfg<64, 64, 2, 2, 16>>::type>::type;
   static constexpr int LDS_BYTES = C::LDS_DOUBLES * 8;
 };
+// FP32 class: FP32 storage, split-BF16 (three-product) MFMA.
 template <int BS>
 struct ClassCfg<kF32, BS> {
   using C = typename std::conditional<BS == 192, LpCfg<192, 192, 2, 4, 32>,
             typename std::conditional<BS == 128, LpCfg<128, 128, 2, 2, 32>, LpCfg<64, 64, 2, 2, 32>>::type>::type;
-  static constexpr int LDS_BYTES = F32Lds<C>::FLOATS * 4;
+  static constexpr int NSPLIT = 3;
+  static constexpr int LDS_BYTES = Bf16sLds<C, NSPLIT>::HALVES * 2;
 };
+// Far class: FP32 storage, single BF16 MFMA operands.
 template <int BS>
 struct ClassCfg<kF32B, BS> {
   using C = typename std::conditional<BS == 192, LpCfg<192, 192, 2, 4, 64>,
             typename std::conditional<BS == 128, LpCfg<128, 128, 2, 2, 64>, LpCfg<64, 64, 2, 2, 64>>::type>::type;
-  static constexpr int LDS_BYTES = Bf16Lds<C>::HALVES * 2;
+  static constexpr int NSPLIT = 1;
+  static constexpr int LDS_BYTES = Bf16sLds<C, NSPLIT>::HALVES * 2;
 };
 
 template <int CLS, int BS, class TA, class TB>
@@ -80,15 +85,10 @@ __device__ __forceinline__ void class_block(const TA* A, const TB* B, void* Cv, 
     zero_acc<C>(acc);
     gemm_nt_acc<C, false, TA, TB>(acc, A, ts, B, ts, ts, reinterpret_cast<double*>(lds), -1.0);
     acc_update<C, double>(acc, static_cast<double*>(Cv), ts, 1.0);
-  } else if constexpr (CLS == kF32) {
-    fx4 acc[C::FM][C::FN];
-    zero_acc_lp<C>(acc);
-    gemm_nt_sub_f32<C, TA, TB>(acc, A, ts, B, ts, ts, reinterpret_cast<float*>(lds));
-    acc_add_lp<C, float>(acc, static_cast<float*>(Cv), ts);
   } else {
     fx4 acc[C::FM][C::FN];
     zero_acc_lp<C>(acc);
-    gemm_nt_sub_bf16<C, TA, TB>(acc, A, ts, B, ts, ts, reinterpret_cast<uint16_t*>(lds));
+    gemm_nt_sub_bf16s<C, Cfg::NSPLIT, TA, TB>(acc, A, ts, B, ts, ts, reinterpret_cast<uint16_t*>(lds));
     acc_add_lp<C, float>(acc, static_cast<float*>(Cv), ts);
   }
 }
