@@ -65,12 +65,23 @@ __device__ __forceinline__ float2 ld_global_f2(const float* p) {
 #endif
 }
 
-// Two consecutive elements of a tile stored as T, widened to FP64.
-__device__ __forceinline__ double2 ld_pair(const double* p) { return ld_global2(p); }
-__device__ __forceinline__ double2 ld_pair(const float* p) {
-  const float2 f = ld_global_f2(p);
-  return double2{(double)f.x, (double)f.y};
-}
+// Two consecutive elements of a tile stored as T, loaded raw (Pair<T>) and
+// widened to FP64 only when stored to LDS, so the loads of the next slice
+// stay in flight while the current one is multiplied.
+template <class T>
+struct Pair;
+template <>
+struct Pair<double> {
+  using type = double2;
+};
+template <>
+struct Pair<float> {
+  using type = float2;
+};
+__device__ __forceinline__ double2 ld_raw(const double* p) { return ld_global2(p); }
+__device__ __forceinline__ float2 ld_raw(const float* p) { return ld_global_f2(p); }
+__device__ __forceinline__ double2 widen(double2 v) { return v; }
+__device__ __forceinline__ double2 widen(float2 v) { return double2{(double)v.x, (double)v.y}; }
 
 template <class C>
 __device__ __forceinline__ void zero_acc(dx4 (&acc)[C::FM][C::FN]) {
@@ -97,8 +108,8 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const TA* 
   const int wn = wave - wm * C::WGN;
   double* As = lds;
   double* Bs = lds + 2 * C::A_STAGE;
-  double2 ra[C::A_LOADS];
-  double2 rb[C::B_LOADS];
+  typename Pair<TA>::type ra[C::A_LOADS];
+  typename Pair<TB>::type rb[C::B_LOADS];
 
   auto gload = [&](int k0) {
 #pragma unroll
@@ -107,11 +118,11 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const TA* 
       if (A_KMAJOR) {
         const int r = e / (C::BK / 2);
         const int k2 = e - r * (C::BK / 2);
-        ra[t] = ld_pair(A + (long)r * lda + k0 + 2 * k2);
+        ra[t] = ld_raw(A + (long)r * lda + k0 + 2 * k2);
       } else {
         const int c = e / (C::BM / 2);
         const int r2 = e - c * (C::BM / 2);
-        ra[t] = ld_pair(A + (long)(k0 + c) * lda + 2 * r2);
+        ra[t] = ld_raw(A + (long)(k0 + c) * lda + 2 * r2);
       }
     }
 #pragma unroll
@@ -120,11 +131,11 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const TA* 
       if (B_KMAJOR) {
         const int c = e / (C::BK / 2);
         const int k2 = e - c * (C::BK / 2);
-        rb[t] = ld_pair(B + (long)c * ldb + k0 + 2 * k2);
+        rb[t] = ld_raw(B + (long)c * ldb + k0 + 2 * k2);
       } else {
         const int c = e / (C::BN / 2);
         const int r2 = e - c * (C::BN / 2);
-        rb[t] = ld_pair(B + (long)(k0 + c) * ldb + 2 * r2);
+        rb[t] = ld_raw(B + (long)(k0 + c) * ldb + 2 * r2);
       }
     }
   };
@@ -135,12 +146,13 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const TA* 
       if (A_KMAJOR) {
         const int r = e / (C::BK / 2);
         const int k2 = e - r * (C::BK / 2);
-        As[buf * C::A_STAGE + (2 * k2) * C::LDA + r] = ra[t].x;
-        As[buf * C::A_STAGE + (2 * k2 + 1) * C::LDA + r] = ra[t].y;
+        const double2 w = widen(ra[t]);
+        As[buf * C::A_STAGE + (2 * k2) * C::LDA + r] = w.x;
+        As[buf * C::A_STAGE + (2 * k2 + 1) * C::LDA + r] = w.y;
       } else {
         const int c = e / (C::BM / 2);
         const int r2 = e - c * (C::BM / 2);
-        *reinterpret_cast<double2*>(As + buf * C::A_STAGE + c * C::LDA + 2 * r2) = ra[t];
+        *reinterpret_cast<double2*>(As + buf * C::A_STAGE + c * C::LDA + 2 * r2) = widen(ra[t]);
       }
     }
 #pragma unroll
@@ -149,12 +161,13 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const TA* 
       if (B_KMAJOR) {
         const int c = e / (C::BK / 2);
         const int k2 = e - c * (C::BK / 2);
-        Bs[buf * C::B_STAGE + (2 * k2) * C::LDB + c] = rb[t].x;
-        Bs[buf * C::B_STAGE + (2 * k2 + 1) * C::LDB + c] = rb[t].y;
+        const double2 w = widen(rb[t]);
+        Bs[buf * C::B_STAGE + (2 * k2) * C::LDB + c] = w.x;
+        Bs[buf * C::B_STAGE + (2 * k2 + 1) * C::LDB + c] = w.y;
       } else {
         const int c = e / (C::BN / 2);
         const int r2 = e - c * (C::BN / 2);
-        *reinterpret_cast<double2*>(Bs + buf * C::B_STAGE + c * C::LDB + 2 * r2) = rb[t];
+        *reinterpret_cast<double2*>(Bs + buf * C::B_STAGE + c * C::LDB + 2 * r2) = widen(rb[t]);
       }
     }
   };

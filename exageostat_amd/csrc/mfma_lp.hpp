@@ -117,27 +117,31 @@ __device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi, uint3
   }
 }
 
-// Loads one 4-row x 8-k unit (column-major source, leading dimension ld)
-// as BF16 rows (hi) and, for NSPLIT = 3, the BF16 remainders (lo);
-// negate flips the sign bits of both (exact).
-template <int NSPLIT, class T>
-__device__ __forceinline__ void load_unit_split(const T* base, long ld, bool negate, uint4 (&hi)[4], uint4 (&lo)[4]) {
-  float v[4][8];
+// One 4-row x 8-k unit of a column-major source (leading dimension ld):
+// eight 16-byte column pieces, loaded raw (FP32, or FP64 narrowed on
+// load) so that the loads of the next slice stay in flight while the
+// current slice is multiplied; the BF16 conversion happens at store time.
+template <class T>
+__device__ __forceinline__ void load_unit_raw(const T* base, long ld, float4 (&q)[8]) {
 #pragma unroll
-  for (int kk = 0; kk < 8; ++kk) {
-    const float4 q = ld_quad(base + (long)kk * ld);
-    v[0][kk] = q.x;
-    v[1][kk] = q.y;
-    v[2][kk] = q.z;
-    v[3][kk] = q.w;
-  }
+  for (int kk = 0; kk < 8; ++kk) q[kk] = ld_quad(base + (long)kk * ld);
+}
+
+// The unit as BF16 rows (hi) and, for NSPLIT = 3, the BF16 remainders
+// (lo); negate flips the sign bits of both (exact).
+template <int NSPLIT>
+__device__ __forceinline__ void split_unit(const float4 (&q)[8], bool negate, uint4 (&hi)[4], uint4 (&lo)[4]) {
   const uint32_t sign = negate ? 0x80008000u : 0u;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     uint32_t wh[4], wl[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
-      split_pair<NSPLIT>(v[r][2 * h], v[r][2 * h + 1], wh[h], wl[h]);
+      const float4& a = q[2 * h];
+      const float4& b = q[2 * h + 1];
+      const float fa = r == 0 ? a.x : (r == 1 ? a.y : (r == 2 ? a.z : a.w));
+      const float fb = r == 0 ? b.x : (r == 1 ? b.y : (r == 2 ? b.z : b.w));
+      split_pair<NSPLIT>(fa, fb, wh[h], wl[h]);
       wh[h] ^= sign;
       wl[h] ^= sign;
     }
@@ -155,18 +159,18 @@ __device__ __forceinline__ void gemm_nt_sub_bf16s(fx4 (&acc)[C::FM][C::FN], cons
   const int wave = tid >> 6;
   const int wm = wave / C::WGN;
   const int wn = wave - wm * C::WGN;
-  uint4 rh[L::SLOTS][4], rl[L::SLOTS][4];
+  float4 raw[L::SLOTS][8];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int t = 0; t < L::SLOTS; ++t) {
       const int u = tid + t * C::NT;
       if (u < L::A_UNITS) {
         const int mq = u % (C::BM / 4), kg = u / (C::BM / 4);
-        load_unit_split<NSPLIT>(A + (long)(k0 + 8 * kg) * lda + 4 * mq, lda, true, rh[t], rl[t]);
+        load_unit_raw(A + (long)(k0 + 8 * kg) * lda + 4 * mq, lda, raw[t]);
       } else if (u < L::A_UNITS + L::B_UNITS) {
         const int w = u - L::A_UNITS;
         const int mq = w % (C::BN / 4), kg = w / (C::BN / 4);
-        load_unit_split<NSPLIT>(B + (long)(k0 + 8 * kg) * ldb + 4 * mq, ldb, false, rh[t], rl[t]);
+        load_unit_raw(B + (long)(k0 + 8 * kg) * ldb + 4 * mq, ldb, raw[t]);
       }
     }
   };
@@ -178,24 +182,27 @@ __device__ __forceinline__ void gemm_nt_sub_bf16s(fx4 (&acc)[C::FM][C::FN], cons
       int row, kg;
       uint16_t* hi_arr;
       uint16_t* lo_arr;
+      uint4 rh[4], rl[4];
       if (u < L::A_UNITS) {
         row = 4 * (u % (C::BM / 4));
         kg = u / (C::BM / 4);
         hi_arr = st;
         lo_arr = st + L::A_ARR + L::B_ARR;
+        split_unit<NSPLIT>(raw[t], true, rh, rl);
       } else if (u < L::A_UNITS + L::B_UNITS) {
         const int w = u - L::A_UNITS;
         row = 4 * (w % (C::BN / 4));
         kg = w / (C::BN / 4);
         hi_arr = st + L::A_ARR;
         lo_arr = st + 2 * L::A_ARR + L::B_ARR;
+        split_unit<NSPLIT>(raw[t], false, rh, rl);
       } else {
         continue;
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        *reinterpret_cast<uint4*>(hi_arr + (row + r) * L::LDK + 8 * kg) = rh[t][r];
-        if (NSPLIT == 3) *reinterpret_cast<uint4*>(lo_arr + (row + r) * L::LDK + 8 * kg) = rl[t][r];
+        *reinterpret_cast<uint4*>(hi_arr + (row + r) * L::LDK + 8 * kg) = rh[r];
+        if (NSPLIT == 3) *reinterpret_cast<uint4*>(lo_arr + (row + r) * L::LDK + 8 * kg) = rl[r];
       }
     }
   };
