@@ -155,7 +155,7 @@ def main() -> None:
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
-              "(see DESIGN.md for how the path would shard)", file=sys.stderr)
+              "(the tile layout shards block-cyclically over a GPU grid; one GPU is measured here)", file=sys.stderr)
         sys.exit(2)
 
     import exageostat_amd

@@ -134,8 +134,7 @@ def load() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
-            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
-            "(or `make -C exageostat_amd`); there is no CPU fallback")
+            f"{LIB_PATH} is missing: build it with `make -C exageostat_amd`; there is no CPU fallback")
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in _SIGNATURES.items():
         fn = getattr(lib, name)

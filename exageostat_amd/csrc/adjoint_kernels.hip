@@ -10,7 +10,7 @@
 //
 // Takahashi recurrence.  With Sigma = L L^T, Z L = L^-T is upper
 // triangular, which for tile column k (from the last to the first) gives
-//   P_jk = L_jk X_kk                       (X_kk = L_kk^-1, j > k)
+//   P_jk = L_jk X_kk                       (X_kk = L_kk^-1, j > k; kept as P_jk^T)
 //   Z_mk = - sum_{j>k} Z(m,j) P_jk         (m > k; Z(m,j) = Z_jm^T for j > m)
 //   Z_kk = X_kk^T X_kk - sum_{j>k} Z_jk^T P_jk.
 // Z overwrites L in place (lower tiles; diagonal tiles full).
@@ -25,7 +25,8 @@ using C192 = GemmCfg<192, 192, 2, 4, 16>;
 using C128 = GemmCfg<128, 128, 2, 2, 16>;
 using CT64 = GemmCfg<64, 64, 2, 2, 16>;
 
-// P_j = L_jk X_kk for j in (k, mt); P holds mt tiles of ts x ts, indexed by j.
+// P_j = L_jk X_kk for j in (k, mt), stored transposed (tile j of P holds
+// P_j^T, column-major), so the two consumers read it row-major.
 template <class C>
 __global__ void __launch_bounds__(C::NT) k_trmm_panel(TileView v, int k, const double* X, double* P) {
   __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
@@ -42,7 +43,7 @@ __global__ void __launch_bounds__(C::NT) k_trmm_panel(TileView v, int k, const d
   dx4 acc[C::FM][C::FN];
   zero_acc<C>(acc);
   gemm_nt_acc<C, true>(acc, A, ts, B, ts, (int)(ts - k0), lds, 1.0);
-  acc_update<C>(acc, P + (long)j * ts * ts + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
+  acc_store_t<C>(acc, P + (long)j * ts * ts + bj * C::BN + (long)bi * C::BM * ts, ts);
 }
 
 // Z_mk = - sum_{j>k} Z(m,j) P_j for m in (k, mt).
@@ -58,13 +59,13 @@ __global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, cons
   dx4 acc[C::FM][C::FN];
   zero_acc<C>(acc);
   for (int j = k + 1; j < v.mt; ++j) {
-    const double* B = P + (long)j * ts * ts + (long)bj * C::BN * ts;  // B(c, kk) = P_j[kk, bj*BN + c]
+    const double* B = P + (long)j * ts * ts + bj * C::BN;  // B(c, kk) = P_j^T[bj*BN + c, kk]
     if (j <= m) {
       const double* A = tile_f64(v, m, j) + bi * C::BM;
-      gemm_nt_acc<C, true>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
+      gemm_nt_acc<C>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
     } else {
       const double* A = tile_f64(v, j, m) + (long)bi * C::BM * ts;  // A(r, kk) = Z_jm[kk, bi*BM + r]
-      gemm_nt_acc<C, true, double, double, true>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
+      gemm_nt_acc<C, false, double, double, true>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
     }
   }
   acc_update<C>(acc, tile_f64(v, m, k) + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
@@ -90,9 +91,8 @@ __global__ void __launch_bounds__(C::NT) k_takahashi_diag(TileView v, int k, con
                                                X + k0 + (long)bj * C::BN * ts, ts, (int)(ts - k0), lds, 1.0);
   } else {
     const int j = k + i;
-    gemm_nt_acc<C, true, double, double, true>(acc, tile_f64(v, j, k) + (long)bi * C::BM * ts, ts,
-                                               P + (long)j * ts * ts + (long)bj * C::BN * ts, ts, (int)ts, lds,
-                                               -1.0);
+    gemm_nt_acc<C, false, double, double, true>(acc, tile_f64(v, j, k) + (long)bi * C::BM * ts, ts,
+                                                P + (long)j * ts * ts + bj * C::BN, ts, (int)ts, lds, -1.0);
   }
   acc_update<C>(acc, Q + (long)i * ts * ts + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
 }

@@ -224,6 +224,26 @@ __device__ __forceinline__ void acc_update(const dx4 (&acc)[C::FM][C::FN], T* Cp
     }
 }
 
+// Cp[c + r*ldc] = acc(r, c): the block stored transposed.
+template <class C>
+__device__ __forceinline__ void acc_store_t(const dx4 (&acc)[C::FM][C::FN], double* Cp, long ldc) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave / C::WGN;
+  const int wn = wave - wm * C::WGN;
+#pragma unroll
+  for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < C::FN; ++j) {
+      const int c = wn * C::WN + j * 16 + frag_col(lane);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = wm * C::WM + i * 16 + frag_row(lane, v);
+        Cp[c + (long)r * ldc] = acc[i][j][v];
+      }
+    }
+}
+
 template <class C, class T = double>
 __device__ __forceinline__ void acc_load(dx4 (&acc)[C::FM][C::FN], const T* Cp, long ldc) {
   const int lane = threadIdx.x & 63;

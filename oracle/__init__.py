@@ -2,8 +2,9 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product (the ``exageostat_amd``
 package, its HIP library or its C-ABI) may import, link or call anything
-under ``oracle/``.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` use it, and only as the checker.
+under ``oracle/``.  Only ``tests/``, the repository entry point's
+``smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` use it, and only
+as the checker.
 
 It restates, in numpy/scipy (floating point) and plain Python/C (integer
 and RNG work), what the reference computes on this path:
