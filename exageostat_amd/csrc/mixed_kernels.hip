@@ -93,14 +93,34 @@ __device__ __forceinline__ void class_block(const TA* A, const TB* B, void* Cv, 
   }
 }
 
-// Persistent update over the class-CLS tile pairs pairs[0..npairs) (packed
-// m << 16 | n), all blocks of each pair; same XCD-aware split of the block
-// list as the FP64 update.
+// One BM x BN block (bi, bj) of C(m, n) -= A(m, k) A(n, k)^T with tile
+// (m, n) of precision class CLS; the operands may be FP64 or FP32 tiles.
 template <int CLS, int BS>
-__global__ void __launch_bounds__((ClassCfg<CLS, BS>::C::NT))
-    k_update_class(TileView v, int k, const int* pairs, long nblocks, int nx) {
+__device__ __forceinline__ void class_update(const TileView& v, int k, int m, int n, int bi, int bj,
+                                             unsigned char* lds) {
   using C = typename ClassCfg<CLS, BS>::C;
-  __shared__ __attribute__((aligned(16))) unsigned char lds[ClassCfg<CLS, BS>::LDS_BYTES];
+  const long ta = tile_index(v, m, k), tb = tile_index(v, n, k), tc = tile_index(v, m, n);
+  const bool fa = v.prec[ta] != kF64, fb = v.prec[tb] != kF64;
+  const long ao = bi * C::BM, bo = bj * C::BN;
+  const long co = bi * C::BM + (long)bj * C::BN * v.ts;
+  void* Cv = (CLS == kF64) ? (void*)(static_cast<double*>(v.ptr[tc]) + co)
+                           : (void*)(static_cast<float*>(v.ptr[tc]) + co);
+  if (fa) {
+    const float* A = static_cast<const float*>(v.ptr[ta]) + ao;
+    if (fb) class_block<CLS, BS>(A, static_cast<const float*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
+    else class_block<CLS, BS>(A, static_cast<const double*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
+  } else {
+    const double* A = static_cast<const double*>(v.ptr[ta]) + ao;
+    if (fb) class_block<CLS, BS>(A, static_cast<const float*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
+    else class_block<CLS, BS>(A, static_cast<const double*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
+  }
+}
+
+// Persistent loop over nblocks blocks of the tile pairs pairs[] (packed
+// m << 16 | n, q*q blocks each): work-group w runs on XCD w % nx and the
+// XCDs take contiguous slices of the block list, as in the FP64 update.
+template <int BS, class F>
+__device__ __forceinline__ void for_each_block(const TileView& v, const int* pairs, long nblocks, int nx, F&& f) {
   const int q = v.ts / BS;
   const int w = blockIdx.x;
   const int x = w % nx;
@@ -116,23 +136,18 @@ __global__ void __launch_bounds__((ClassCfg<CLS, BS>::C::NT))
     const int pk = pairs[p];
     const int m = pk >> 16;
     const int n = pk & 0xFFFF;
-    if (m == n && (bi + 1) * C::BM <= bj * C::BN) continue;
-    const long ta = tile_index(v, m, k), tb = tile_index(v, n, k), tc = tile_index(v, m, n);
-    const bool fa = v.prec[ta] != kF64, fb = v.prec[tb] != kF64;
-    const long ao = bi * C::BM, bo = bj * C::BN;
-    const long co = bi * C::BM + (long)bj * C::BN * v.ts;
-    void* Cv = (CLS == kF64) ? (void*)(static_cast<double*>(v.ptr[tc]) + co)
-                             : (void*)(static_cast<float*>(v.ptr[tc]) + co);
-    if (fa) {
-      const float* A = static_cast<const float*>(v.ptr[ta]) + ao;
-      if (fb) class_block<CLS, BS>(A, static_cast<const float*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
-      else class_block<CLS, BS>(A, static_cast<const double*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
-    } else {
-      const double* A = static_cast<const double*>(v.ptr[ta]) + ao;
-      if (fb) class_block<CLS, BS>(A, static_cast<const float*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
-      else class_block<CLS, BS>(A, static_cast<const double*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
-    }
+    if (m == n && bi < bj) continue;  // strictly upper block of a diagonal tile
+    f(m, n, bi, bj);
   }
+}
+
+// Update over the pairs of one precision class CLS.
+template <int CLS, int BS>
+__global__ void __launch_bounds__((ClassCfg<CLS, BS>::C::NT))
+    k_update_class(TileView v, int k, const int* pairs, long nblocks, int nx) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[ClassCfg<CLS, BS>::LDS_BYTES];
+  for_each_block<BS>(v, pairs, nblocks, nx,
+                     [&](int m, int n, int bi, int bj) { class_update<CLS, BS>(v, k, m, n, bi, bj, lds); });
 }
 
 template <int CLS, int BS>

@@ -83,15 +83,28 @@ def fit(m, lb, ub, start=None, method: str = "lbfgsb", opt_tol: int = 5, max_eva
         history.append((np.array(th), float(val)))
         return _PENALTY if not np.isfinite(val) else -val / n
 
+    good = {"u": None, "f": None, "penalized": False}
+
     def f_and_g(u):
-        th = np.clip(to_theta(np.asarray(u)), lb, ub)
+        u = np.asarray(u, dtype=float)
+        th = np.clip(to_theta(u), lb, ub)
         val, g = m.loglik(th, grad=True)
         counts["evals"] += 1
         counts["grads"] += 1
         history.append((np.array(th), float(val)))
         if not np.isfinite(val) or not np.all(np.isfinite(g)):
-            return _PENALTY, np.zeros_like(th)
-        return -val / n, -np.asarray(g) * jac_scale(th) / n
+            # Sigma(theta) not positive definite: a smooth bowl around the
+            # last feasible point, above every value seen there, so that the
+            # line search backtracks towards it instead of stalling.
+            good["penalized"] = True
+            if good["u"] is None:
+                return _PENALTY, np.zeros_like(u)
+            scale = 1.0 + abs(good["f"])
+            d = u - good["u"]
+            return good["f"] + scale * (1.0 + 1e3 * float(d @ d)), 2e3 * scale * d
+        f = -val / n
+        good["u"], good["f"] = u.copy(), f
+        return f, -np.asarray(g) * jac_scale(th) / n
 
     t0 = time.perf_counter()
     if method == "dfo":
@@ -103,8 +116,18 @@ def fit(m, lb, ub, start=None, method: str = "lbfgsb", opt_tol: int = 5, max_eva
                                          "final_tr_radius": 10.0 ** (-opt_tol - 1)})
     elif method == "lbfgsb":
         ll_scale = 10.0 ** (-opt_tol) / n
-        res = optimize.minimize(f_and_g, u0, jac=True, method="L-BFGS-B", bounds=list(zip(ulb, uub)),
-                                options={"maxfun": max_evals, "ftol": ll_scale, "gtol": gtol})
+        ustart = u0
+        for _ in range(4):
+            # A run that met an infeasible point may stop early: restart from
+            # the best point with a fresh quasi-Newton memory.
+            good["penalized"] = False
+            res = optimize.minimize(f_and_g, ustart, jac=True, method="L-BFGS-B", bounds=list(zip(ulb, uub)),
+                                    options={"maxfun": max(1, max_evals - counts["evals"]), "ftol": ll_scale,
+                                             "gtol": gtol})
+            if not good["penalized"] or counts["evals"] >= max_evals:
+                break
+            best_th = max(history, key=lambda h: h[1] if np.isfinite(h[1]) else -np.inf)[0]
+            ustart = np.log(best_th) if log_params else best_th
     else:
         raise ValueError(f"unknown method {method!r}")
     wall = time.perf_counter() - t0
