@@ -88,6 +88,16 @@ __device__ __forceinline__ void acc_add_lp(const fx4 (&acc)[C::FM][C::FN], T* Cp
 // distinct banks).  Global loads come in units of 4 rows x 8 k (eight
 // 16-byte column pieces), narrowed and split in registers; A units go to
 // the first threads of the work-group, B units to the next ones.
+#ifndef EXA_LDS_SWIZZLE_SHIFT
+#define EXA_LDS_SWIZZLE_SHIFT 3
+#endif
+// 16-byte k-group kg of row `row` is stored at group kg ^ ((row >> S) & 3):
+// the unit stores write rows 4 apart from consecutive lanes, which without
+// the swizzle fall on the same LDS banks (shift 3 measured best on gfx950);
+// the fragment reads (16 consecutive
+// rows, one k-group) stay conflict-free.
+__device__ __forceinline__ int lds_kgroup(int row, int kg) { return kg ^ ((row >> EXA_LDS_SWIZZLE_SHIFT) & 3); }
+
 template <class C, int NSPLIT>
 struct Bf16sLds {
   static_assert(NSPLIT == 1 || NSPLIT == 3, "one or three BF16 products");
@@ -201,8 +211,9 @@ __device__ __forceinline__ void gemm_nt_sub_bf16s(fx4 (&acc)[C::FM][C::FN], cons
       }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        *reinterpret_cast<uint4*>(hi_arr + (row + r) * L::LDK + 8 * kg) = rh[r];
-        if (NSPLIT == 3) *reinterpret_cast<uint4*>(lo_arr + (row + r) * L::LDK + 8 * kg) = rl[r];
+        const int off = (row + r) * L::LDK + 8 * lds_kgroup(row + r, kg);
+        *reinterpret_cast<uint4*>(hi_arr + off) = rh[r];
+        if (NSPLIT == 3) *reinterpret_cast<uint4*>(lo_arr + off) = rl[r];
       }
     }
   };
@@ -214,24 +225,33 @@ __device__ __forceinline__ void gemm_nt_sub_bf16s(fx4 (&acc)[C::FM][C::FN], cons
     const int buf = kc & 1;
     if (kc + 1 < nk) gload((kc + 1) * C::BK);
     const uint16_t* st = lds + buf * L::STAGE;
-    const int aoff = (wm * C::WM + (lane & 15)) * L::LDK + 8 * (lane >> 4);
-    const int boff = (wn * C::WN + (lane & 15)) * L::LDK + 8 * (lane >> 4);
 #pragma unroll
     for (int kk = 0; kk < C::BK; kk += 32) {
+      const int kg = kk / 8 + (lane >> 4);
+      int ao[C::FM], bo[C::FN];
+#pragma unroll
+      for (int i = 0; i < C::FM; ++i) {
+        const int row = wm * C::WM + i * 16 + (lane & 15);
+        ao[i] = row * L::LDK + 8 * lds_kgroup(row, kg);
+      }
+#pragma unroll
+      for (int j = 0; j < C::FN; ++j) {
+        const int row = wn * C::WN + j * 16 + (lane & 15);
+        bo[j] = row * L::LDK + 8 * lds_kgroup(row, kg);
+      }
       bf16x8 ah[C::FM], bh[C::FN];
 #pragma unroll
-      for (int i = 0; i < C::FM; ++i) ah[i] = *reinterpret_cast<const bf16x8*>(st + aoff + i * 16 * L::LDK + kk);
+      for (int i = 0; i < C::FM; ++i) ah[i] = *reinterpret_cast<const bf16x8*>(st + ao[i]);
 #pragma unroll
-      for (int j = 0; j < C::FN; ++j)
-        bh[j] = *reinterpret_cast<const bf16x8*>(st + L::A_ARR + boff + j * 16 * L::LDK + kk);
+      for (int j = 0; j < C::FN; ++j) bh[j] = *reinterpret_cast<const bf16x8*>(st + L::A_ARR + bo[j]);
       if (NSPLIT == 3) {
         bf16x8 al[C::FM], bl[C::FN];
 #pragma unroll
         for (int i = 0; i < C::FM; ++i)
-          al[i] = *reinterpret_cast<const bf16x8*>(st + L::A_ARR + L::B_ARR + aoff + i * 16 * L::LDK + kk);
+          al[i] = *reinterpret_cast<const bf16x8*>(st + L::A_ARR + L::B_ARR + ao[i]);
 #pragma unroll
         for (int j = 0; j < C::FN; ++j)
-          bl[j] = *reinterpret_cast<const bf16x8*>(st + 2 * L::A_ARR + L::B_ARR + boff + j * 16 * L::LDK + kk);
+          bl[j] = *reinterpret_cast<const bf16x8*>(st + 2 * L::A_ARR + L::B_ARR + bo[j]);
         // small cross terms first, then the leading product
 #pragma unroll
         for (int i = 0; i < C::FM; ++i)
