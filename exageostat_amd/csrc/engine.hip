@@ -372,6 +372,12 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
     const double below = (double)A.m - (double)(k + 1) * ts;
     if (k + 1 < mt)
       profiled("trsm_panel", below * rk * rk, 0.0, s1, [&] { launch_trsm_panel(v, k, ws.dinv(k), s1); });
+    // The BF16 update classes read the panel in split BF16 form.  The buffer
+    // of step k is reused at step k + 2, whose panel (on s1) waits for the
+    // column update of step k + 1, queued on s2 after the rest of step k.
+    if (lists && k + 1 < mt)
+      profiled("split_panel", 0.0, 12.0 * below * A.ts, s1,
+               [&] { launch_split_panel(v, k, lists->split[k & 1], s1); });
   };
   auto update = [&](int k, int lo, int hi, hipStream_t s) {
     if (!lists) {
@@ -384,8 +390,9 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
       const long end = lists->off[c][std::min(hi + 1, mt)];
       if (end <= beg) continue;
       const double fl = (lists->flops[c][end] - lists->flops[c][beg]) * rows(k);
-      profiled(names[c], fl, 0.0, s,
-               [&] { launch_update_class(c, v, k, lists->d_pairs[c] + beg, end - beg, s); });
+      profiled(names[c], fl, 0.0, s, [&] {
+        launch_update_class(c, v, k, lists->d_pairs[c] + beg, end - beg, lists->split[k & 1], s);
+      });
     }
   };
   // s1 carries the panel factorisations (small kernels on the CUs the

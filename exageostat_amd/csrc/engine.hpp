@@ -125,10 +125,13 @@ class CholWorkspace {
 // class of tile (m, n): pairs[c] (device, packed m << 16 | n) sorted by
 // column then row, off[c][n] = index of the first pair of column >= n,
 // flops[c] = prefix sums of the pairs' algorithmic update flops per step.
+// split[2]: BF16 parts of the panels of even / odd steps (owned by the
+// caller, mt tiles of 2 ts^2 BF16 values each).
 struct ClassLists {
   int* d_pairs[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<long> off[4];
   std::vector<double> flops[4];
+  uint16_t* split[2] = {nullptr, nullptr};
   ~ClassLists();
   void build(const TileMatrix& A);
 };

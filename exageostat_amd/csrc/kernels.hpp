@@ -77,9 +77,14 @@ void launch_zero_upper_diag(const TileView& v, hipStream_t s);
 // ---- mixed precision ---------------------------------------------------
 // dst <- src tile by tile (dst tiles FP64 or FP32 by dst.prec).
 void launch_convert_mixed(const TileView& src, const TileView& dst, long ntiles, hipStream_t s);
+// BF16 hi/lo parts of the panel tiles (m, k), m > k, for the BF16 update
+// classes: tile m at split + 2 m ts^2 (mt tiles of room).
+void launch_split_panel(const TileView& v, int k, uint16_t* split, hipStream_t s);
 // Trailing update of step k for the tile pairs (packed m << 16 | n) of one
-// precision class (kF64, kF32 or kF32B) of the updated tile.
-void launch_update_class(int cls, const TileView& v, int k, const int* pairs, long npairs, hipStream_t s);
+// precision class (kF64, kF32 or kF32B) of the updated tile; the BF16
+// classes read their operands from the split panel of step k.
+void launch_update_class(int cls, const TileView& v, int k, const int* pairs, long npairs, const uint16_t* split,
+                         hipStream_t s);
 // out = a*x + b*y (n entries).
 void launch_axpby(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s);
 // Transposed half of y = Sigma x for a lower FP64 matrix (see mixed_kernels).

@@ -29,6 +29,8 @@ namespace exa {
 struct MixedState {
   TileMatrix Cm;       // mixed factor, every tile in an FP64-sized slot
   ClassLists lists;    // update pairs per precision class
+  uint16_t* split = nullptr;  // two panels in split BF16 form
+  long split_tiles = 0;
   DeviceVector x, r;   // solution and correction
   double* part = nullptr;  // partial sums of Sigma x: 2 * mt columns of mt*ts
   long part_len = 0;
@@ -41,6 +43,7 @@ struct MixedState {
   int bbox_ts = 0;
   ~MixedState() {
     if (part) (void)hipFree(part);
+    if (split) (void)hipFree(split);
   }
 };
 
@@ -162,8 +165,16 @@ double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric,
     EXA_HIP(hipMalloc(&st.part, sizeof(double) * 2L * mt * padded));
     st.part_len = 2L * mt * padded;
   }
+  if (st.split_tiles < mt) {
+    if (st.split) (void)hipFree(st.split);
+    st.split = nullptr;
+    EXA_HIP(hipMalloc(&st.split, sizeof(uint16_t) * 4L * mt * ts * ts));
+    st.split_tiles = mt;
+  }
   st.Cm.set_prec(precision_map(st, ctx, kid, theta, metric, opt, mt, ts));
   st.lists.build(st.Cm);
+  st.lists.split[0] = st.split;
+  st.lists.split[1] = st.split + 2L * mt * ts * ts;
   profiled("gen", 0.0, 4.0 * (double)N * (N + 1.0), s1,
            [&] { generate_covariance(ctx.C, kid, theta, ctx.l1, ctx.l1, metric, s1); });
   profiled("convert", 0.0, 0.0, s1,

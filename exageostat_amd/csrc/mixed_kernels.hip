@@ -16,6 +16,7 @@
 #include "mfma_lp.hpp"
 
 #include <cstdlib>
+#include <stdexcept>
 #include <type_traits>
 
 namespace exa {
@@ -48,6 +49,46 @@ void launch_convert_mixed(const TileView& src, const TileView& dst, long ntiles,
   EXA_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------- panel split --
+// Panel tiles (m, k), m in (k, mt), converted once per step into BF16 hi and
+// lo parts for the BF16 classes: tile m of `split` holds hi at [0, ts^2) and
+// lo at [ts^2, 2 ts^2), each laid out [k-group g][row r][8] (the layout the
+// update kernels copy straight into LDS).  One thread per (row, group).
+__global__ void __launch_bounds__(256) k_split_panel(TileView v, int k, uint16_t* split) {
+  const int ts = v.ts;
+  const long per_tile = (long)ts * (ts / 8);
+  const long u = (long)blockIdx.x * 256 + threadIdx.x;
+  const int m = k + 1 + (int)(u / per_tile);
+  if (m >= v.mt) return;
+  const long w = u - (long)(m - k - 1) * per_tile;
+  const int g = (int)(w / ts);
+  const int r = (int)(w - (long)g * ts);
+  const long t = tile_index(v, m, k);
+  float f[8];
+  if (v.prec[t] == kF64) {
+    const double* T = static_cast<const double*>(v.ptr[t]);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) f[e] = (float)T[r + (long)(8 * g + e) * ts];
+  } else {
+    const float* T = static_cast<const float*>(v.ptr[t]);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) f[e] = T[r + (long)(8 * g + e) * ts];
+  }
+  uint32_t hi[4], lo[4];
+#pragma unroll
+  for (int h = 0; h < 4; ++h) split_pair<3>(f[2 * h], f[2 * h + 1], hi[h], lo[h]);
+  uint16_t* dst = split + (long)m * 2 * ts * ts + ((long)g * ts + r) * 8;
+  *reinterpret_cast<uint4*>(dst) = uint4{hi[0], hi[1], hi[2], hi[3]};
+  *reinterpret_cast<uint4*>(dst + (long)ts * ts) = uint4{lo[0], lo[1], lo[2], lo[3]};
+}
+
+void launch_split_panel(const TileView& v, int k, uint16_t* split, hipStream_t s) {
+  const long units = (long)(v.mt - k - 1) * v.ts * (v.ts / 8);
+  if (units <= 0) return;
+  hipLaunchKernelGGL(k_split_panel, dim3((unsigned)((units + 255) / 256)), dim3(256), 0, s, v, k, split);
+  EXA_HIP(hipGetLastError());
+}
+
 // --------------------------------------------------------- class updates --
 // One block of C(m,n) -= A(m,k) B(n,k)^T, C of precision class CLS.
 template <int CLS, int BS>
@@ -65,7 +106,7 @@ struct ClassCfg<kF32, BS> {
   using C = typename std::conditional<BS == 192, LpCfg<192, 192, 2, 4, 32>,
             typename std::conditional<BS == 128, LpCfg<128, 128, 2, 2, 32>, LpCfg<64, 64, 2, 2, 32>>::type>::type;
   static constexpr int NSPLIT = 3;
-  static constexpr int LDS_BYTES = Bf16sLds<C, NSPLIT>::HALVES * 2;
+  static constexpr int LDS_BYTES = Bf16pLds<C, NSPLIT>::HALVES * 2;
 };
 // Far class: FP32 storage, single BF16 MFMA operands.
 template <int BS>
@@ -73,46 +114,53 @@ struct ClassCfg<kF32B, BS> {
   using C = typename std::conditional<BS == 192, LpCfg<192, 192, 2, 4, 64>,
             typename std::conditional<BS == 128, LpCfg<128, 128, 2, 2, 64>, LpCfg<64, 64, 2, 2, 64>>::type>::type;
   static constexpr int NSPLIT = 1;
-  static constexpr int LDS_BYTES = Bf16sLds<C, NSPLIT>::HALVES * 2;
+  static constexpr int LDS_BYTES = Bf16pLds<C, NSPLIT>::HALVES * 2;
 };
 
-template <int CLS, int BS, class TA, class TB>
-__device__ __forceinline__ void class_block(const TA* A, const TB* B, void* Cv, int ts, unsigned char* lds) {
-  using Cfg = ClassCfg<CLS, BS>;
-  using C = typename Cfg::C;
-  if constexpr (CLS == kF64) {
-    dx4 acc[C::FM][C::FN];
-    zero_acc<C>(acc);
-    gemm_nt_acc<C, false, TA, TB>(acc, A, ts, B, ts, ts, reinterpret_cast<double*>(lds), -1.0);
-    acc_update<C, double>(acc, static_cast<double*>(Cv), ts, 1.0);
-  } else {
-    fx4 acc[C::FM][C::FN];
-    zero_acc_lp<C>(acc);
-    gemm_nt_sub_bf16s<C, Cfg::NSPLIT, TA, TB>(acc, A, ts, B, ts, ts, reinterpret_cast<uint16_t*>(lds));
-    acc_add_lp<C, float>(acc, static_cast<float*>(Cv), ts);
-  }
+// FP64 class: one block of C(m,n) -= A(m,k) B(n,k)^T on the FP64 matrix
+// cores, operands read from the tiles (FP64 or FP32 storage).
+template <int BS, class TA, class TB>
+__device__ __forceinline__ void f64_block(const TA* A, const TB* B, double* Cp, int ts, unsigned char* lds) {
+  using C = typename ClassCfg<kF64, BS>::C;
+  dx4 acc[C::FM][C::FN];
+  zero_acc<C>(acc);
+  gemm_nt_acc<C, false, TA, TB>(acc, A, ts, B, ts, ts, reinterpret_cast<double*>(lds), -1.0);
+  acc_update<C, double>(acc, Cp, ts, 1.0);
 }
 
 // One BM x BN block (bi, bj) of C(m, n) -= A(m, k) A(n, k)^T with tile
-// (m, n) of precision class CLS; the operands may be FP64 or FP32 tiles.
+// (m, n) of precision class CLS.  The FP64 class reads the panel tiles; the
+// BF16 classes read the panel's BF16 parts from `split` (tile j at
+// split + 2 j ts^2, see k_split_panel).
 template <int CLS, int BS>
 __device__ __forceinline__ void class_update(const TileView& v, int k, int m, int n, int bi, int bj,
-                                             unsigned char* lds) {
+                                             const uint16_t* split, unsigned char* lds) {
   using C = typename ClassCfg<CLS, BS>::C;
-  const long ta = tile_index(v, m, k), tb = tile_index(v, n, k), tc = tile_index(v, m, n);
-  const bool fa = v.prec[ta] != kF64, fb = v.prec[tb] != kF64;
-  const long ao = bi * C::BM, bo = bj * C::BN;
+  const long tc = tile_index(v, m, n);
   const long co = bi * C::BM + (long)bj * C::BN * v.ts;
-  void* Cv = (CLS == kF64) ? (void*)(static_cast<double*>(v.ptr[tc]) + co)
-                           : (void*)(static_cast<float*>(v.ptr[tc]) + co);
-  if (fa) {
-    const float* A = static_cast<const float*>(v.ptr[ta]) + ao;
-    if (fb) class_block<CLS, BS>(A, static_cast<const float*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
-    else class_block<CLS, BS>(A, static_cast<const double*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
+  if constexpr (CLS == kF64) {
+    const long ta = tile_index(v, m, k), tb = tile_index(v, n, k);
+    const bool fa = v.prec[ta] != kF64, fb = v.prec[tb] != kF64;
+    const long ao = bi * C::BM, bo = bj * C::BN;
+    double* Cp = static_cast<double*>(v.ptr[tc]) + co;
+    if (fa) {
+      const float* A = static_cast<const float*>(v.ptr[ta]) + ao;
+      if (fb) f64_block<BS>(A, static_cast<const float*>(v.ptr[tb]) + bo, Cp, v.ts, lds);
+      else f64_block<BS>(A, static_cast<const double*>(v.ptr[tb]) + bo, Cp, v.ts, lds);
+    } else {
+      const double* A = static_cast<const double*>(v.ptr[ta]) + ao;
+      if (fb) f64_block<BS>(A, static_cast<const float*>(v.ptr[tb]) + bo, Cp, v.ts, lds);
+      else f64_block<BS>(A, static_cast<const double*>(v.ptr[tb]) + bo, Cp, v.ts, lds);
+    }
   } else {
-    const double* A = static_cast<const double*>(v.ptr[ta]) + ao;
-    if (fb) class_block<CLS, BS>(A, static_cast<const float*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
-    else class_block<CLS, BS>(A, static_cast<const double*>(v.ptr[tb]) + bo, Cv, v.ts, lds);
+    const long tt = (long)v.ts * v.ts;
+    const uint16_t* Ah = split + 2 * (long)m * tt + (long)bi * C::BM * 8;
+    const uint16_t* Bh = split + 2 * (long)n * tt + (long)bj * C::BN * 8;
+    fx4 acc[C::FM][C::FN];
+    zero_acc_lp<C>(acc);
+    gemm_nt_bf16p<C, ClassCfg<CLS, BS>::NSPLIT>(acc, Ah, Ah + tt, Bh, Bh + tt, v.ts, v.ts,
+                                                reinterpret_cast<uint16_t*>(lds));
+    acc_add_lp<C, float>(acc, static_cast<float*>(v.ptr[tc]) + co, v.ts, -1.f);
   }
 }
 
@@ -144,10 +192,10 @@ __device__ __forceinline__ void for_each_block(const TileView& v, const int* pai
 // Update over the pairs of one precision class CLS.
 template <int CLS, int BS>
 __global__ void __launch_bounds__((ClassCfg<CLS, BS>::C::NT))
-    k_update_class(TileView v, int k, const int* pairs, long nblocks, int nx) {
+    k_update_class(TileView v, int k, const int* pairs, long nblocks, int nx, const uint16_t* split) {
   __shared__ __attribute__((aligned(16))) unsigned char lds[ClassCfg<CLS, BS>::LDS_BYTES];
   for_each_block<BS>(v, pairs, nblocks, nx,
-                     [&](int m, int n, int bi, int bj) { class_update<CLS, BS>(v, k, m, n, bi, bj, lds); });
+                     [&](int m, int n, int bi, int bj) { class_update<CLS, BS>(v, k, m, n, bi, bj, split, lds); });
 }
 
 template <int CLS, int BS>
@@ -169,29 +217,33 @@ static int class_grid_cap() {
 }
 
 template <int CLS, int BS>
-static void launch_class_bs(const TileView& v, int k, const int* pairs, long npairs, hipStream_t s) {
+static void launch_class_bs(const TileView& v, int k, const int* pairs, long npairs, const uint16_t* split,
+                            hipStream_t s) {
   const long q = v.ts / BS;
   const long blocks = npairs * q * q;
   const int cap = class_grid_cap<CLS, BS>();
   const int grid = blocks < cap ? (int)blocks : cap;
   const int nx = (grid % 8 == 0) ? 8 : 1;
   hipLaunchKernelGGL((k_update_class<CLS, BS>), dim3((unsigned)grid), dim3(ClassCfg<CLS, BS>::C::NT), 0, s, v, k,
-                     pairs, blocks, nx);
+                     pairs, blocks, nx, split);
   EXA_HIP(hipGetLastError());
 }
 
 template <int CLS>
-static void launch_class(const TileView& v, int k, const int* pairs, long npairs, hipStream_t s) {
-  if (v.ts % 192 == 0) launch_class_bs<CLS, 192>(v, k, pairs, npairs, s);
-  else if (v.ts % 128 == 0) launch_class_bs<CLS, 128>(v, k, pairs, npairs, s);
-  else launch_class_bs<CLS, 64>(v, k, pairs, npairs, s);
+static void launch_class(const TileView& v, int k, const int* pairs, long npairs, const uint16_t* split,
+                         hipStream_t s) {
+  if (v.ts % 192 == 0) launch_class_bs<CLS, 192>(v, k, pairs, npairs, split, s);
+  else if (v.ts % 128 == 0) launch_class_bs<CLS, 128>(v, k, pairs, npairs, split, s);
+  else launch_class_bs<CLS, 64>(v, k, pairs, npairs, split, s);
 }
 
-void launch_update_class(int cls, const TileView& v, int k, const int* pairs, long npairs, hipStream_t s) {
+void launch_update_class(int cls, const TileView& v, int k, const int* pairs, long npairs, const uint16_t* split,
+                         hipStream_t s) {
   if (npairs <= 0) return;
-  if (cls == kF64) launch_class<kF64>(v, k, pairs, npairs, s);
-  else if (cls == kF32) launch_class<kF32>(v, k, pairs, npairs, s);
-  else launch_class<kF32B>(v, k, pairs, npairs, s);
+  if (cls != kF64 && split == nullptr) throw std::invalid_argument("BF16 update classes need the split panel");
+  if (cls == kF64) launch_class<kF64>(v, k, pairs, npairs, split, s);
+  else if (cls == kF32) launch_class<kF32>(v, k, pairs, npairs, split, s);
+  else launch_class<kF32B>(v, k, pairs, npairs, split, s);
 }
 
 // ------------------------------------------------ symmetric product (IR) --
