@@ -130,6 +130,117 @@ def _rate(p, key):
     return p[key] / (p["ms"] * 1e-3), p["ms"] / max(p["launches"], 1)
 
 
+def _fp64_vs_mixed(m, theta, steps: int, mixed_opts: dict) -> dict:
+    """Times FP64 and mixed evaluations of one ExactMLE (FP64 first, then the
+    mixed path on the same data) and compares l and log det."""
+    from exageostat_amd import mle
+    out = {}
+    for path in ("fp64", "mixed"):
+        if path == "mixed":
+            m.precision = 2
+            m.set_mixed(**mixed_opts)
+        else:
+            m.precision = 0
+        m.loglik(theta)  # warm-up
+        mle.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ll = m.loglik(theta)
+        dt = (time.perf_counter() - t0) / steps
+        out[path] = {"s_per_eval": dt, "evals_per_s": 1.0 / dt, "loglik": ll, "logdet": m.logdet, "dotp": m.dotp,
+                     "status": m.status}
+        print(f"bench: {path} {dt:.3f}s/eval l={ll:.10f}", file=sys.stderr, flush=True)
+    a, b = out["fp64"], out["mixed"]
+    out["rel_dl"] = abs(b["loglik"] - a["loglik"]) / abs(a["loglik"])
+    out["dlogdet"] = b["logdet"] - a["logdet"]
+    out["rel_dlogdet"] = abs(b["logdet"] - a["logdet"]) / abs(a["logdet"])
+    out["rel_ddotp"] = abs(b["dotp"] - a["dotp"]) / abs(a["dotp"])
+    out["speedup"] = a["s_per_eval"] / b["s_per_eval"]
+    m.precision = 0
+    return out
+
+
+def run_c3(steps: int) -> None:
+    """Mixed-precision accuracy and speed at n = 160,000 (400 x 400 jittered
+    grid), ts = 1,024, for theta = (1, 0.03, 1.0) and (1, 0.3, 0.5); z drawn
+    at each theta; FP64 diagonal tiles, FP32 tiles updated with split BF16,
+    BF16 far tiles (correlation < 0.01), 3 FP64 refinement steps."""
+    import exageostat_amd
+    from exageostat_amd import mle
+    n, ts = 160000, 1024
+    x, y = exageostat_amd.generate_xy_loc(n, 0)
+    e, _ = mle.dlarnv(3, [0, 0, 0, 1], n)
+    m = exageostat_amd.ExactMLE(x, y, ts)
+    res = []
+    for theta in ([1.0, 0.03, 1.0], [1.0, 0.3, 0.5]):
+        m.zvg(theta, e)
+        r = _fp64_vs_mixed(m, theta, steps, MIXED)
+        m.precision = 2
+        m.set_mixed(**MIXED)
+        m.loglik(theta)
+        pm = m.prec_map()
+        low = pm[np.tril_indices(pm.shape[0])]
+        r["tiles"] = {"f64": int(np.sum(low == 0)), "f32": int(np.sum(low == 1)), "bf16": int(np.sum(low == 3))}
+        m.precision = 0
+        r["theta"] = theta
+        res.append(r)
+    m.close()
+    print(json.dumps({"config": "C3: n=160000, ts=1024, mixed vs FP64", "mixed_options": MIXED, "results": res}),
+          flush=True)
+
+
+def run_c5(steps: int) -> None:
+    """Parsimonious bivariate Matern, 50,000 locations x 2 variables
+    (n = 100,000 rows; 224 x 224 jittered grid truncated), theta = (1, 1, 0.1,
+    0.5, 1.0, 0.5): FP64 vs mixed l, the six-parameter adjoint gradient with
+    central differences (h = 1e-6 relative) for two of its components, and
+    kriging of 2,000 held-out locations per variable (seeded random split of
+    the locations; both sets keep the Morton order)."""
+    import exageostat_amd
+    from exageostat_amd import mle
+    nloc, ts = 50000, 960
+    kern = "bivariate_matern_parsimonious"
+    theta = np.array([1.0, 1.0, 0.1, 0.5, 1.0, 0.5])
+    x, y = exageostat_amd.generate_xy_loc(nloc, 0)
+    e, _ = mle.dlarnv(3, [0, 0, 0, 1], 2 * nloc)
+    m = exageostat_amd.ExactMLE(x, y, ts, kernel=kern)
+    m.zvg(theta, e)
+    z = m.get_z()
+    out = {"config": "C5: bivariate parsimonious, 50000 locations x 2", "theta": theta.tolist()}
+    out["fp64_vs_mixed"] = _fp64_vs_mixed(m, theta, steps, MIXED)
+    t0 = time.perf_counter()
+    _, g = m.loglik(theta, grad=True)
+    out["grad_s"] = time.perf_counter() - t0
+    out["grad"] = list(g)
+    fd = {}
+    for p in (4, 5):
+        h = 1e-6 * abs(theta[p])
+        tp, tm = theta.copy(), theta.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd[p] = (m.loglik(tp) - m.loglik(tm)) / (2 * h)
+    out["grad_fd_check"] = {str(p): {"adjoint": g[p], "central_fd": v, "rel": abs(g[p] - v) / abs(v)}
+                            for p, v in fd.items()}
+    m.close()
+    print(f"bench: gradient {out['grad_s']:.1f}s", file=sys.stderr, flush=True)
+    # kriging: 2,000 held-out locations (both variables)
+    zz = z.reshape(nloc, 2)
+    perm = np.random.default_rng(0).permutation(nloc)
+    miss, obs = np.sort(perm[:2000]), np.sort(perm[2000:])
+    xo, yo, xm, ym = x[obs], y[obs], x[miss], y[miss]
+    zo = zz[obs].ravel()
+    za = zz[miss].ravel()
+    m = exageostat_amd.ExactMLE(xo, yo, ts, kernel=kern)
+    t0 = time.perf_counter()
+    zm, mse = m.predict(theta, xo, yo, zo, xm, ym, za)
+    out["kriging"] = {"s": time.perf_counter() - t0, "n_obs_locations": len(xo), "n_miss_locations": len(xm),
+                      "mse_var1": m.data.mserror1, "mse_var2": m.data.mserror2,
+                      "mse_check_var1": float(np.mean((za[0::2] - zm[0::2]) ** 2)),
+                      "mse_check_var2": float(np.mean((za[1::2] - zm[1::2]) ** 2))}
+    m.close()
+    print(json.dumps(out), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +251,8 @@ def main() -> None:
     ap.add_argument("--cpu-sample", type=int, default=16000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="time the FP64 path only")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c5"],
+                    help="c2 (default): the benchmark line; c3, c5: the other configurations' measurements")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy kernel of BYTES (scales the HBM counters) and exit")
     args = ap.parse_args()
@@ -157,6 +270,13 @@ def main() -> None:
         print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
               "(the tile layout shards block-cyclically over a GPU grid; one GPU is measured here)", file=sys.stderr)
         sys.exit(2)
+
+    if args.config == "c3":
+        run_c3(args.steps)
+        return
+    if args.config == "c5":
+        run_c5(args.steps)
+        return
 
     import exageostat_amd
     from exageostat_amd import mle
