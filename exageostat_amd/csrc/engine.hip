@@ -273,6 +273,8 @@ CholWorkspace::~CholWorkspace() {
   if (dinv_) (void)hipFree(dinv_);
   if (xinv_) (void)hipFree(xinv_);
   if (yvec_) (void)hipFree(yvec_);
+  if (tpart_) (void)hipFree(tpart_);
+  if (ticket_) (void)hipFree(ticket_);
   if (info_) (void)hipFree(info_);
   if (h_info_) (void)hipHostFree(h_info_);
   for (auto e : ev_) (void)hipEventDestroy(e);
@@ -283,12 +285,21 @@ void CholWorkspace::ensure(int mt, int ts) {
   if (dinv_) (void)hipFree(dinv_);
   if (xinv_) (void)hipFree(xinv_);
   if (yvec_) (void)hipFree(yvec_);
-  dinv_ = xinv_ = yvec_ = nullptr;
+  if (tpart_) (void)hipFree(tpart_);
+  dinv_ = xinv_ = yvec_ = tpart_ = nullptr;
   mt_ = mt;
   ts_ = ts;
   EXA_HIP(hipMalloc(&dinv_, sizeof(double) * (size_t)mt * (ts / kNB) * 4096));
   EXA_HIP(hipMalloc(&xinv_, sizeof(double) * (size_t)mt * ts * ts));
   EXA_HIP(hipMalloc(&yvec_, sizeof(double) * (size_t)mt * ts));
+  EXA_HIP(hipMalloc(&tpart_, sizeof(double) * (size_t)(ts / kNB) * ts));
+  if (!ticket_) {
+    EXA_HIP(hipMalloc(&ticket_, sizeof(unsigned)));
+    // synchronous on the null stream, which does not order with the
+    // non-blocking streams the kernels run on: wait for it here
+    EXA_HIP(hipMemset(ticket_, 0, sizeof(unsigned)));
+    EXA_HIP(hipDeviceSynchronize());
+  }
   if (!info_) EXA_HIP(hipMalloc(&info_, sizeof(int)));
   if (!h_info_) EXA_HIP(hipHostMalloc(&h_info_, sizeof(int), hipHostMallocDefault));
 }
@@ -439,12 +450,12 @@ void tile_trsv(const TileMatrix& L, const CholWorkspace& ws, double* z, int tran
   double* y = ws.yvec();
   if (!trans) {
     for (int k = 0; k < L.mt; ++k) {
-      launch_trmv_diag(ws.xinv(k), ts, z + (long)k * ts, y + (long)k * ts, 0, s);
+      launch_trmv_diag(ws.xinv(k), ts, z + (long)k * ts, y + (long)k * ts, 0, ws.tpart(), ws.ticket(), s);
       launch_gemv_panel(v, k, y + (long)k * ts, z, s);
     }
   } else {
     for (int k = L.mt - 1; k >= 0; --k) {
-      launch_trmv_diag(ws.xinv(k), ts, z + (long)k * ts, y + (long)k * ts, 1, s);
+      launch_trmv_diag(ws.xinv(k), ts, z + (long)k * ts, y + (long)k * ts, 1, ws.tpart(), ws.ticket(), s);
       launch_gemv_t_row(v, k, y + (long)k * ts, z, s);
     }
   }

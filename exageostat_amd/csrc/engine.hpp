@@ -105,6 +105,10 @@ class CholWorkspace {
   double* xinv_base() const { return xinv_; }
   // Scratch vector of mt*ts doubles for the triangular solves.
   double* yvec() const { return yvec_; }
+  // Partial sums ((ts/64) * ts doubles) and the zeroed completion ticket of
+  // the diagonal-tile products in the solves.
+  double* tpart() const { return tpart_; }
+  unsigned* ticket() const { return ticket_; }
   int* info() const { return info_; }      // device
   int* host_info() const { return h_info_; }  // pinned host mirror
   hipEvent_t ev(int i);
@@ -116,6 +120,8 @@ class CholWorkspace {
   double* dinv_ = nullptr;
   double* xinv_ = nullptr;
   double* yvec_ = nullptr;
+  double* tpart_ = nullptr;
+  unsigned* ticket_ = nullptr;
   int* info_ = nullptr;
   int* h_info_ = nullptr;
   std::vector<hipEvent_t> ev_;

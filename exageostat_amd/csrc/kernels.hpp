@@ -53,8 +53,11 @@ void launch_tile_update(const TileView& v, int k, int col_lo, int col_hi, hipStr
 void launch_trtri_tiles(const TileView& v, const double* dinv, double* xinv, hipStream_t s);
 
 // ---- vectors --------------------------------------------------------------
-// y = X t (trans = 0) or X^T t (trans = 1) for one lower ts x ts tile X.
-void launch_trmv_diag(const double* X, int ts, const double* t, double* y, int trans, hipStream_t s);
+// y = X t (trans = 0) or X^T t (trans = 1) for one lower ts x ts tile X;
+// part: (ts/64) * ts doubles of scratch, ticket: a zeroed device counter
+// (left zeroed), both private to the stream.
+void launch_trmv_diag(const double* X, int ts, const double* t, double* y, int trans, double* part,
+                      unsigned* ticket, hipStream_t s);
 // z_m -= L(m,k) y_k for m in (k, mt).
 void launch_gemv_panel(const TileView& v, int k, const double* yk, double* z, hipStream_t s);
 // z_m -= L(k,m)^T y_k for m in [0, k).

@@ -62,23 +62,26 @@ __device__ __forceinline__ void acc_add_lp(const fx4 (&acc)[C::FM][C::FN], T* Cp
 // ------------------------------------------------------- split BF16 --
 // The BF16 update classes multiply BF16 operands on the matrix cores
 // (v_mfma_f32_16x16x32_bf16, FP32 accumulation).  NSPLIT = 1: operands
-// rounded to BF16.  NSPLIT = 3: each operand is split as x = hi + lo
-// (hi = bf16(x), lo = bf16(x - hi)) and the product is hi*hi + hi*lo +
-// lo*hi, which keeps about 16 significant bits of each operand (products of
-// BF16 values are exact in FP32) at one third of the BF16 rate, several
-// times the FP32 MFMA rate.
+// rounded to BF16.  NSPLIT = 3 or 4: each operand is split as x = hi + lo
+// (hi = bf16(x), lo = bf16(x - hi), about 16 significant bits together) and
+// the product is hi*hi + hi*lo + lo*hi (+ lo*lo for NSPLIT = 4); products of
+// BF16 values are exact in FP32.  The lo*lo term is small (2^-16 relative)
+// but not random: rows of neighbouring locations have strongly correlated
+// remainders, so dropping it biases A*B^T, and through the Schur complements
+// the log-determinant, in one direction.  NSPLIT = 4 runs at a quarter of
+// the BF16 rate, several times the FP32 MFMA rate.
 
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef __bf16 b2v __attribute__((ext_vector_type(2)));
 
 // Two FP32 values rounded to BF16 (hardware round-to-nearest-even, one
-// packed conversion) and, for NSPLIT = 3, their BF16 remainders.
+// packed conversion) and, for NSPLIT >= 3, their BF16 remainders.
 template <int NSPLIT>
 __device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi, uint32_t& lo) {
   const f2v v = {a, b};
   const b2v h = __builtin_convertvector(v, b2v);
   hi = __builtin_bit_cast(uint32_t, h);
-  if (NSPLIT == 3) {
+  if (NSPLIT >= 3) {
     const f2v back = __builtin_convertvector(h, f2v);
     lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(v - back, b2v));
   }
@@ -96,7 +99,7 @@ struct Bf16pLds {
   static_assert(C::BK % 32 == 0, "BK must be a multiple of the MFMA depth");
   static constexpr int LDK = C::BK + 8;
   static constexpr int A_ARR = C::BM * LDK, B_ARR = C::BN * LDK;
-  static constexpr int NARR = NSPLIT == 3 ? 2 : 1;
+  static constexpr int NARR = NSPLIT >= 3 ? 2 : 1;
   static constexpr int STAGE = NARR * (A_ARR + B_ARR);
   static constexpr int HALVES = 2 * STAGE;
   static constexpr int G = C::BK / 8;                          // k-groups per slice
@@ -183,7 +186,7 @@ __device__ __forceinline__ void gemm_nt_bf16p(fx4 (&acc)[C::FM][C::FN], const ui
       for (int j = 0; j < C::FN; ++j)
         bh[j] = *reinterpret_cast<const bf16x8*>(st + L::A_ARR + (wn * C::WN + j * 16 + (lane & 15)) * L::LDK +
                                                  kcol);
-      if (NSPLIT == 3) {
+      if (NSPLIT >= 3) {
         bf16x8 al[C::FM], bl[C::FN];
         const uint16_t* sl = st + L::A_ARR + L::B_ARR;
 #pragma unroll
@@ -197,6 +200,9 @@ __device__ __forceinline__ void gemm_nt_bf16p(fx4 (&acc)[C::FM][C::FN], const ui
         for (int i = 0; i < C::FM; ++i)
 #pragma unroll
           for (int j = 0; j < C::FN; ++j) {
+            // smallest terms first
+            if (NSPLIT == 4)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bl[j], acc[i][j], 0, 0, 0);
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
           }

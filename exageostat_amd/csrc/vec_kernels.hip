@@ -20,61 +20,98 @@ __device__ inline double wave_sum(double v) {
 }
 
 // sum over columns c = g, g+4, ... < ts of A[c*ts] * y[c] (A points at one
-// row of a column-major tile).
+// row of a column-major tile); eight loads in flight per step.
 template <class T>
 __device__ inline double row_dot(const T* A, int ts, const double* y, int g) {
-  double s0 = 0.0, s1 = 0.0;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
   int c = g;
-  for (; c + 4 < ts; c += 8) {
-    s0 += (double)A[(long)c * ts] * y[c];
-    s1 += (double)A[(long)(c + 4) * ts] * y[c + 4];
+  for (; c + 28 < ts; c += 32) {
+    T a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] = A[(long)(c + 4 * u) * ts];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s[u & 3] += (double)a[u] * y[c + 4 * u];
   }
-  for (; c < ts; c += 4) s0 += (double)A[(long)c * ts] * y[c];
-  return s0 + s1;
+  for (; c < ts; c += 4) s[0] += (double)A[(long)c * ts] * y[c];
+  return (s[0] + s[1]) + (s[2] + s[3]);
 }
 
-// this lane's share (rows r, r+64, ...) of sum_rows col[row] * y[row].
+// Sums of 16 consecutive columns of a column-major tile against y:
+// s[j] = sum over rows of A[row + j*ts] * y[row], this lane taking rows
+// r, r + 64, ... < ts; the caller reduces over the wave.  16 loads in
+// flight per step.
 template <class T>
-__device__ inline double col_dot(const T* col, int ts, const double* y, int r) {
-  double s = 0.0;
-  for (int row = r; row < ts; row += 64) s += (double)col[row] * y[row];
-  return s;
+__device__ inline void col_dot16(const T* A, int ts, const double* y, int r, double (&s)[16]) {
+#pragma unroll
+  for (int j = 0; j < 16; ++j) s[j] = 0.0;
+  for (int row = r; row < ts; row += 64) {
+    T a[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) a[j] = A[row + (long)j * ts];
+    const double yr = y[row];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) s[j] += (double)a[j] * yr;
+  }
 }
 
-// y = X t (trans = 0) or y = X^T t (trans = 1) for one lower-triangular
-// ts x ts tile X (the inverse of a diagonal factor tile); one work-group
-// per 64 rows (trans = 0) or 64 columns (trans = 1).
-__global__ void __launch_bounds__(256) k_trmv_diag(const double* X, int ts, const double* t, double* y,
-                                                   int trans) {
-  __shared__ double part[4][64];
+// Partial products of one lower-triangular ts x ts tile X (the inverse of
+// a diagonal factor tile) with t, one work-group per 64 x 64 block (rb, cb)
+// of its lower block triangle (q(q+1)/2 blocks, q = ts / 64):
+// trans = 0: part[cb][rb*64 + i] = sum_j X[rb*64+i][cb*64+j] t[cb*64+j];
+// trans = 1: part[rb][cb*64 + j] = sum_i X[rb*64+i][cb*64+j] t[rb*64+i].
+// The last work-group to finish (an atomic ticket after a device-scope
+// fence) sums the partials in a fixed order into y and resets the ticket,
+// so the result does not depend on the work-group schedule.
+__global__ void __launch_bounds__(256) k_trmv_diag(const double* X, int ts, const double* t, double* y, int trans,
+                                                   double* part, unsigned* ticket) {
+  __shared__ double red[4][64];
+  __shared__ int last;
   const int tid = threadIdx.x;
   const int r = tid & 63;
   const int g = tid >> 6;
-  const int b = blockIdx.x;
+  const int q = ts / kNB;
+  int rb, cb;
+  tri_decode(blockIdx.x, rb, cb);
+  const double* B = X + (long)(cb * kNB + g * 16) * ts + rb * kNB;
+  double a[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) a[j] = B[r + (long)j * ts];
   if (!trans) {
-    const int row = b * kNB + r;
-    const int cend = (b + 1) * kNB;
     double s0 = 0.0, s1 = 0.0;
-    int c = g;
-    for (; c + 4 < cend; c += 8) {
-      s0 += X[row + (long)c * ts] * t[c];
-      s1 += X[row + (long)(c + 4) * ts] * t[c + 4];
+#pragma unroll
+    for (int j = 0; j < 16; j += 2) {
+      s0 += a[j] * t[cb * kNB + g * 16 + j];
+      s1 += a[j + 1] * t[cb * kNB + g * 16 + j + 1];
     }
-    for (; c < cend; c += 4) s0 += X[row + (long)c * ts] * t[c];
-    part[g][r] = s0 + s1;
+    red[g][r] = s0 + s1;
     __syncthreads();
-    if (tid < 64) y[row] = (part[0][r] + part[1][r]) + (part[2][r] + part[3][r]);
+    if (tid < 64) part[(long)cb * ts + rb * kNB + r] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
   } else {
-    for (int cc = g * 16; cc < g * 16 + 16; ++cc) {
-      const int col = b * kNB + cc;
-      double s = 0.0;
-      for (int row = b * kNB + r; row < ts; row += 64) s += X[row + (long)col * ts] * t[row];
-      s = wave_sum(s);
-      if (r == 0) part[0][cc] = s;
+    const double tr = t[rb * kNB + r];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) a[j] = wave_sum(a[j] * tr);
+    if (r == 0) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) part[(long)rb * ts + cb * kNB + g * 16 + j] = a[j];
     }
-    __syncthreads();
-    if (tid < 64) y[b * kNB + r] = part[0][r];
   }
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  for (int i = tid; i < ts; i += 256) {
+    const int b = i / kNB;
+    double s = 0.0;
+    if (!trans) {
+      for (int c = 0; c <= b; ++c) s += part[(long)c * ts + i];
+    } else {
+      for (int rr = b; rr < q; ++rr) s += part[(long)rr * ts + i];
+    }
+    y[i] = s;
+  }
+  if (tid == 0) *ticket = 0u;
 }
 
 // z_m -= L(m,k) y_k for m in (k, mt); one work-group per 64-row block.
@@ -103,10 +140,10 @@ __global__ void __launch_bounds__(256) k_gemv_panel(TileView v, int k, const dou
   }
 }
 
-// z_m -= L(k,m)^T y_k for m in [0, k); one work-group per 64 columns of L(k,m).
+// z_m -= L(k,m)^T y_k for m in [0, k); one work-group per 64 columns of
+// L(k,m), 16 columns per wave.
 __global__ void __launch_bounds__(256) k_gemv_t_row(TileView v, int k, const double* yk, double* z) {
   __shared__ double y[1024];
-  __shared__ double part[64];
   const int tid = threadIdx.x;
   const int r = tid & 63;
   const int g = tid >> 6;
@@ -117,16 +154,19 @@ __global__ void __launch_bounds__(256) k_gemv_t_row(TileView v, int k, const dou
   for (int i = tid; i < ts; i += 256) y[i] = yk[i];
   __syncthreads();
   const long ti = tile_index(v, k, m);
-  const bool f32 = v.prec && v.prec[ti] != kF64;
-  for (int cc = g * 16; cc < g * 16 + 16; ++cc) {
-    const int col = cb * kNB + cc;
-    double s = f32 ? col_dot(static_cast<const float*>(v.ptr[ti]) + (long)col * ts, ts, y, r)
-                   : col_dot(static_cast<const double*>(v.ptr[ti]) + (long)col * ts, ts, y, r);
-    s = wave_sum(s);
-    if (r == 0) part[cc] = s;
+  const long c0 = (long)(cb * kNB + g * 16) * ts;
+  double s[16];
+  if (v.prec && v.prec[ti] != kF64)
+    col_dot16(static_cast<const float*>(v.ptr[ti]) + c0, ts, y, r, s);
+  else
+    col_dot16(static_cast<const double*>(v.ptr[ti]) + c0, ts, y, r, s);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) s[j] = wave_sum(s[j]);
+  if (r == 0) {
+    double* zc = z + (long)m * ts + cb * kNB + g * 16;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) zc[j] -= s[j];
   }
-  __syncthreads();
-  if (tid < 64) z[(long)m * ts + cb * kNB + r] -= part[r];
 }
 
 __global__ void __launch_bounds__(256) k_logdet(TileView v, double* partial) {
@@ -211,8 +251,10 @@ void launch_zero_upper_diag(const TileView& v, hipStream_t s) {
   EXA_HIP(hipGetLastError());
 }
 
-void launch_trmv_diag(const double* X, int ts, const double* t, double* y, int trans, hipStream_t s) {
-  hipLaunchKernelGGL(k_trmv_diag, dim3(ts / kNB), dim3(256), 0, s, X, ts, t, y, trans);
+void launch_trmv_diag(const double* X, int ts, const double* t, double* y, int trans, double* part,
+                      unsigned* ticket, hipStream_t s) {
+  const int q = ts / kNB;
+  hipLaunchKernelGGL(k_trmv_diag, dim3(q * (q + 1) / 2), dim3(256), 0, s, X, ts, t, y, trans, part, ticket);
   EXA_HIP(hipGetLastError());
 }
 
