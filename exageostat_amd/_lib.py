@@ -58,7 +58,7 @@ class MLEData(ctypes.Structure):
         ("status", ctypes.c_int), ("logdet", ctypes.c_double), ("time_gen", ctypes.c_double),
         ("time_facto", ctypes.c_double), ("time_logdet", ctypes.c_double), ("time_solve", ctypes.c_double),
         ("time_total", ctypes.c_double), ("mp_fp32_corr", ctypes.c_double), ("mp_bf16", ctypes.c_int),
-        ("mp_ir_steps", ctypes.c_int), ("mp_fp64_corr", ctypes.c_double),
+        ("mp_ir_steps", ctypes.c_int), ("mp_fp64_corr", ctypes.c_double), ("mp_lolo", ctypes.c_int),
     ]
 
 

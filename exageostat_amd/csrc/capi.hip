@@ -740,6 +740,7 @@ double EXAGEOSTAT_sdmle_Tile(unsigned n, const double* theta, double* grad, void
     opt.bf16 = data->mp_bf16 != 0;
     opt.ir_steps = data->mp_ir_steps;
     opt.diag_thick = data->diag_thick;
+    opt.lolo = data->mp_lolo != 0;
     double logdet = 0.0, dotp = 0.0;
     int info = 0;
     double loglik = mixed_loglik(*ctx, kid, theta, metric_of(data->dm), opt, &logdet, &dotp, &info);

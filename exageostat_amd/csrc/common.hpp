@@ -55,6 +55,13 @@ __device__ inline double* tile_f64(const TileView& v, int m, int n) {
   return static_cast<double*>(v.ptr[tile_index(v, m, n)]);
 }
 
+// Sum over the 64 lanes of a wave (every lane gets the total).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 // Row index r of the lower-triangle enumeration p = r(r+1)/2 + c, 0<=c<=r.
 __device__ inline void tri_decode(long p, int& r, int& c) {
   long rr = (long)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);

@@ -402,7 +402,7 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
       if (end <= beg) continue;
       const double fl = (lists->flops[c][end] - lists->flops[c][beg]) * rows(k);
       profiled(names[c], fl, 0.0, s, [&] {
-        launch_update_class(c, v, k, lists->d_pairs[c] + beg, end - beg, lists->split[k & 1], s);
+        launch_update_class(c, v, k, lists->d_pairs[c] + beg, end - beg, lists->split[k & 1], lists->lolo, s);
       });
     }
   };

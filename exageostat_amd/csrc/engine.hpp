@@ -138,6 +138,7 @@ struct ClassLists {
   std::vector<long> off[4];
   std::vector<double> flops[4];
   uint16_t* split[2] = {nullptr, nullptr};
+  bool lolo = false;  // kF32 updates with four split-BF16 products
   ~ClassLists();
   void build(const TileMatrix& A);
 };

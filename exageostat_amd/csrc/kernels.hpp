@@ -85,13 +85,15 @@ void launch_convert_mixed(const TileView& src, const TileView& dst, long ntiles,
 void launch_split_panel(const TileView& v, int k, uint16_t* split, hipStream_t s);
 // Trailing update of step k for the tile pairs (packed m << 16 | n) of one
 // precision class (kF64, kF32 or kF32B) of the updated tile; the BF16
-// classes read their operands from the split panel of step k.
+// classes read their operands from the split panel of step k.  lolo: the
+// kF32 class adds the lo*lo product (four split-BF16 products).
 void launch_update_class(int cls, const TileView& v, int k, const int* pairs, long npairs, const uint16_t* split,
-                         hipStream_t s);
+                         bool lolo, hipStream_t s);
 // out = a*x + b*y (n entries).
 void launch_axpby(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s);
-// Transposed half of y = Sigma x for a lower FP64 matrix (see mixed_kernels).
-void launch_gemv_t_full_partial(const TileView& v, const double* x, double* part, hipStream_t s);
+// y = Sigma x for a lower-packed FP64 matrix with full diagonal tiles, each
+// tile read once; part: 2 * mt * (mt * ts) doubles of scratch.
+void launch_symv(const TileView& v, const double* x, double* y, double* part, hipStream_t s);
 
 // ---- adjoint gradient ---------------------------------------------------
 struct MaternGradParams {

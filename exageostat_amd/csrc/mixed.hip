@@ -139,11 +139,7 @@ static std::vector<uint8_t> precision_map(MixedState& st, const Context& ctx, Ke
 
 // y = Sigma x for the FP64 lower Sigma held in ctx.C (full diagonal tiles).
 static void symv(const TileMatrix& C, const double* x, double* y, double* part, hipStream_t s) {
-  const TileView v = C.view();
-  const long len = (long)C.mt * C.ts;
-  launch_gemv_full_partial(v, x, part, s);
-  launch_gemv_t_full_partial(v, x, part, s);
-  launch_sum_columns(part, 2 * C.mt, len, y, (long)C.m, s);
+  launch_symv(C.view(), x, y, part, s);
 }
 
 double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric, const MixedOptions& opt,
@@ -175,6 +171,7 @@ double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric,
   st.lists.build(st.Cm);
   st.lists.split[0] = st.split;
   st.lists.split[1] = st.split + 2L * mt * ts * ts;
+  st.lists.lolo = opt.lolo;
   profiled("gen", 0.0, 4.0 * (double)N * (N + 1.0), s1,
            [&] { generate_covariance(ctx.C, kid, theta, ctx.l1, ctx.l1, metric, s1); });
   profiled("convert", 0.0, 0.0, s1,

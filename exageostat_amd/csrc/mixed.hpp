@@ -12,6 +12,7 @@ struct MixedOptions {
   bool bf16 = false;
   int ir_steps = 3;
   int diag_thick = 1;
+  bool lolo = false;  // FP32 class: hi*hi + hi*lo + lo*hi (+ lo*lo when set)
 };
 
 double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric, const MixedOptions& opt,

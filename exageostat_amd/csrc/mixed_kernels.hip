@@ -108,6 +108,15 @@ struct ClassCfg<kF32, BS> {
   static constexpr int NSPLIT = 3;
   static constexpr int LDS_BYTES = Bf16pLds<C, NSPLIT>::HALVES * 2;
 };
+// FP32 class with the lo*lo product as well (four split-BF16 products); a
+// kernel variant of kF32, not a storage class.
+constexpr int kF32Q = 4;
+template <int BS>
+struct ClassCfg<kF32Q, BS> {
+  using C = typename ClassCfg<kF32, BS>::C;
+  static constexpr int NSPLIT = 4;
+  static constexpr int LDS_BYTES = Bf16pLds<C, NSPLIT>::HALVES * 2;
+};
 // Far class: FP32 storage, single BF16 MFMA operands.
 template <int BS>
 struct ClassCfg<kF32B, BS> {
@@ -238,53 +247,106 @@ static void launch_class(const TileView& v, int k, const int* pairs, long npairs
 }
 
 void launch_update_class(int cls, const TileView& v, int k, const int* pairs, long npairs, const uint16_t* split,
-                         hipStream_t s) {
+                         bool lolo, hipStream_t s) {
   if (npairs <= 0) return;
   if (cls != kF64 && split == nullptr) throw std::invalid_argument("BF16 update classes need the split panel");
   if (cls == kF64) launch_class<kF64>(v, k, pairs, npairs, split, s);
+  else if (cls == kF32 && lolo) launch_class<kF32Q>(v, k, pairs, npairs, split, s);
   else if (cls == kF32) launch_class<kF32>(v, k, pairs, npairs, split, s);
   else launch_class<kF32B>(v, k, pairs, npairs, split, s);
 }
 
 // ------------------------------------------------ symmetric product (IR) --
-// part[(mt + m) * mt*ts + n*ts + col] = (A(m,n)^T x_m)[col] for m > n (zero
-// otherwise), for a lower-packed FP64 matrix whose diagonal tiles are full:
-// together with the A(m,n) x_n partials of k_gemv_full_partial this gives
-// y = Sigma x.  One work-group per (tile n, 64 columns, tile m).
-__global__ void __launch_bounds__(256) k_gemv_t_full_partial(TileView v, const double* x, double* part_out) {
-  __shared__ double part[64];
+// y = Sigma x for a lower-packed FP64 matrix whose diagonal tiles are full.
+// One work-group per lower tile (m, n) reads the tile once for both
+//   partN[n][m*ts + i] = sum_j A(m,n)[i][j] x_n[j]   (m >= n) and
+//   partT[m][n*ts + j] = sum_i A(m,n)[i][j] x_m[i]   (m > n),
+// with partN = part, partT = part + mt*len, len = mt*ts.  Wave g takes a
+// quarter of the columns, 16 at a time (16 loads in flight per lane); lane
+// r takes rows r, r + 64, ...
+__global__ void __launch_bounds__(256) k_symv_tile(TileView v, const double* x, double* part) {
+  __shared__ double xn[1024], xm[1024];
+  __shared__ double red[4][1024];
   const int tid = threadIdx.x;
   const int r = tid & 63;
   const int g = tid >> 6;
   const int ts = v.ts;
   const int q = ts / kNB;
-  const int n = blockIdx.x / q;
-  const int cb = blockIdx.x - n * q;
-  const int m = blockIdx.y;
+  int m, n;
+  tri_decode(blockIdx.x, m, n);
   const long len = (long)v.mt * ts;
-  double* out = part_out + (long)(v.mt + m) * len + (long)n * ts + cb * kNB;
-  if (m <= n) {
-    if (tid < 64) out[r] = 0.0;
-    return;
-  }
-  const double* A = tile_f64(v, m, n);
-  const double* xm = x + (long)m * ts;
-  for (int cc = g * 16; cc < g * 16 + 16; ++cc) {
-    const int col = cb * kNB + cc;
-    double s = 0.0;
-    for (int row = r; row < ts; row += 64) s += A[row + (long)col * ts] * xm[row];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (r == 0) part[cc] = s;
+  for (int i = tid; i < ts; i += 256) {
+    xn[i] = x[(long)n * ts + i];
+    xm[i] = x[(long)m * ts + i];
   }
   __syncthreads();
-  if (tid < 64) out[r] = part[r];
+  const double* A = tile_f64(v, m, n);
+  double racc[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) racc[i] = 0.0;
+  const int cw = ts / 4;
+  for (int c0 = g * cw; c0 < (g + 1) * cw; c0 += 16) {
+    double xc[16], cacc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      xc[j] = xn[c0 + j];
+      cacc[j] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      if (i < q) {
+        const int row = r + kNB * i;
+        double a[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) a[j] = A[row + (long)(c0 + j) * ts];
+        const double xr = xm[row];
+        double rs0 = 0.0, rs1 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; j += 2) {
+          rs0 += a[j] * xc[j];
+          rs1 += a[j + 1] * xc[j + 1];
+          cacc[j] += a[j] * xr;
+          cacc[j + 1] += a[j + 1] * xr;
+        }
+        racc[i] += rs0 + rs1;
+      }
+    }
+    if (m > n) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) cacc[j] = wave_sum(cacc[j]);
+      if (r == 0) {
+        double* out = part + (long)(v.mt + m) * len + (long)n * ts + c0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) out[j] = cacc[j];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if (i < q) red[g][r + kNB * i] = racc[i];
+  __syncthreads();
+  double* outn = part + (long)n * len + (long)m * ts;
+  for (int i = tid; i < ts; i += 256) outn[i] = (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]);
 }
 
-void launch_gemv_t_full_partial(const TileView& v, const double* x, double* part, hipStream_t s) {
-  const int q = v.ts / kNB;
-  hipLaunchKernelGGL(k_gemv_t_full_partial, dim3((unsigned)(v.nt * q), (unsigned)v.mt), dim3(256), 0, s, v, x,
-                     part);
+// y[i] (row i of tile p) = sum_{n <= p} partN[n][i] + sum_{m > p} partT[m][i].
+__global__ void k_symv_reduce(const double* part, int mt, int ts, long rows, double* y) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const long len = (long)mt * ts;
+  const int p = (int)(i / ts);
+  double s = 0.0;
+  for (int n = 0; n <= p; ++n) s += part[(long)n * len + i];
+  for (int m = p + 1; m < mt; ++m) s += part[(long)(mt + m) * len + i];
+  y[i] = s;
+}
+
+void launch_symv(const TileView& v, const double* x, double* y, double* part, hipStream_t s) {
+  if (!v.lower || v.ts > 1024 || v.ts % kNB) throw std::invalid_argument("symv: lower matrix, ts <= 1024, ts % 64 == 0");
+  const long ntiles = (long)v.mt * (v.mt + 1) / 2;
+  hipLaunchKernelGGL(k_symv_tile, dim3((unsigned)ntiles), dim3(256), 0, s, v, x, part);
+  hipLaunchKernelGGL(k_symv_reduce, dim3((unsigned)((v.m + 255) / 256)), dim3(256), 0, s, part, v.mt, v.ts,
+                     (long)v.m, y);
   EXA_HIP(hipGetLastError());
 }
 

@@ -13,12 +13,6 @@
 
 namespace exa {
 
-__device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
 // sum over columns c = g, g+4, ... < ts of A[c*ts] * y[c] (A points at one
 // row of a column-major tile); eight loads in flight per step.
 template <class T>

@@ -152,17 +152,19 @@ class ExactMLE:
         return (val, g) if grad else val
 
     def set_mixed(self, fp32_corr: float = 0.0, bf16: bool = False, fp64_corr: float = 0.0,
-                  diag_thick: int = 1, ir_steps: int = 3) -> None:
+                  diag_thick: int = 1, ir_steps: int = 3, lolo: bool = False) -> None:
         """Mixed-precision options (used when precision == 2): tiles whose
         largest correlation is >= fp64_corr stay FP64, tiles below fp32_corr
         get BF16 updates when bf16 is on, the rest are FP32; the solve is
-        refined ir_steps times in FP64."""
+        refined ir_steps times in FP64.  lolo: the FP32 tiles' split-BF16
+        products keep their lo*lo term (four products instead of three)."""
         d = self.data
         d.mp_fp32_corr = fp32_corr
         d.mp_bf16 = 1 if bf16 else 0
         d.mp_fp64_corr = fp64_corr
         d.diag_thick = diag_thick
         d.mp_ir_steps = ir_steps
+        d.mp_lolo = 1 if lolo else 0
 
     @property
     def precision(self) -> int:

@@ -137,6 +137,9 @@ typedef struct {
   int mp_ir_steps;      /* mixed: FP64 iterative-refinement steps on the solve */
   double mp_fp64_corr;  /* mixed: off-diagonal tiles whose largest
                            correlation is >= this stay FP64                  */
+  int mp_lolo;          /* mixed: 1 adds the lo*lo term to the split-BF16
+                           products of the FP32 tiles (four products instead
+                           of three: smaller log-det bias, ~6% slower)       */
 } MLE_data;
 
 /* ---------------------------------------------------------------- setup -- */

@@ -62,10 +62,11 @@ OPTS = [
     dict(bf16=True, fp32_corr=0.01),                   # weak tiles with BF16 updates
     dict(bf16=True, fp32_corr=0.05, diag_thick=2),     # wider FP64 band
     dict(fp64_corr=0.3, bf16=True, fp32_corr=0.01),    # strongly correlated tiles FP64
+    dict(bf16=True, fp32_corr=0.01, lolo=True),        # four split-BF16 products
 ]
 
 
-@pytest.mark.parametrize("opts", OPTS, ids=["f32", "bf16", "band2", "f64corr"])
+@pytest.mark.parametrize("opts", OPTS, ids=["f32", "bf16", "band2", "f64corr", "lolo"])
 @pytest.mark.parametrize("theta", [(1.0, 0.1, 0.5), (1.0, 0.03, 0.5), (1.5, 0.05, 1.0), (0.8, 0.07, 1.37)])
 def test_mixed_loglik_matches_oracle(lib, field, theta, opts):
     x, y, z = field
@@ -197,3 +198,28 @@ def test_mixed_matches_fp64_gpu_large(lib):
     assert np.any(pm == 3)
     assert abs(val - l64) <= MIXED_TOL * abs(l64), (val, l64)
     m.close()
+
+
+def test_mixed_lolo_reduces_logdet_bias(lib):
+    """n = 30720 (ts = 960), exponential kernel with range 0.3 (every
+    off-diagonal tile FP32): dropping the lo*lo term of the split-BF16
+    products biases log|Sigma| (neighbouring rows have correlated BF16
+    remainders); keeping it must shrink the log-det error against the FP64
+    GPU path, and both stay within the mixed tolerance."""
+    n, ts, theta = 30720, 960, (1.0, 0.3, 0.5)
+    x, y, _ = datagen.generate_xy_loc(n, 0)
+    e, _ = datagen.dlarnv(3, [0, 0, 0, 1], n)
+    ref = lib.ExactMLE(x, y, ts)
+    ref.zvg(theta, e)
+    z = ref.get_z()
+    l64 = ref.loglik(theta)
+    ld64 = ref.logdet
+    ref.close()
+    errs = {}
+    for lolo in (False, True):
+        m = _mixed(lib, x, y, z, ts, bf16=True, fp32_corr=0.01, lolo=lolo)
+        val = m.loglik(theta)
+        assert abs(val - l64) <= MIXED_TOL * abs(l64), (lolo, val, l64)
+        errs[lolo] = abs(m.logdet - ld64)
+        m.close()
+    assert errs[True] < errs[False], errs
