@@ -205,23 +205,39 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const TA* 
 
 // C[r + c*ldc] = beta*C + acc for the block owned by this work-group.
 template <class C, class T = double>
-__device__ __forceinline__ void acc_update(const dx4 (&acc)[C::FM][C::FN], T* Cp, long ldc, double beta) {
+__device__ __forceinline__ void acc_update(dx4 (&acc)[C::FM][C::FN], T* Cp, long ldc, double beta) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm = wave / C::WGN;
   const int wn = wave - wm * C::WGN;
+  // Global-address-space accesses, one fragment row at a time with all its
+  // reads issued before its writes: with generic read-modify-writes the
+  // compiler must assume that a store may alias the next load and
+  // serialises the block's memory traffic.
+  typedef __attribute__((address_space(1))) T gT;
+  gT* G = (gT*)(uintptr_t)Cp;
 #pragma unroll
-  for (int i = 0; i < C::FM; ++i)
+  for (int i = 0; i < C::FM; ++i) {
+    if (beta != 0.0) {
+      T cv[C::FN][4];
+#pragma unroll
+      for (int j = 0; j < C::FN; ++j) {
+        const int c = wn * C::WN + j * 16 + frag_col(lane);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cv[j][v] = G[wm * C::WM + i * 16 + frag_row(lane, v) + (long)c * ldc];
+      }
+#pragma unroll
+      for (int j = 0; j < C::FN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[i][j][v] = beta * (double)cv[j][v] + acc[i][j][v];
+    }
 #pragma unroll
     for (int j = 0; j < C::FN; ++j) {
       const int c = wn * C::WN + j * 16 + frag_col(lane);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = wm * C::WM + i * 16 + frag_row(lane, v);
-        T* p = Cp + r + (long)c * ldc;
-        *p = (T)((beta == 0.0 ? 0.0 : beta * (double)*p) + acc[i][j][v]);
-      }
+      for (int v = 0; v < 4; ++v) G[wm * C::WM + i * 16 + frag_row(lane, v) + (long)c * ldc] = (T)acc[i][j][v];
     }
+  }
 }
 
 // Cp[c + r*ldc] = acc(r, c): the block stored transposed.
