@@ -3,8 +3,8 @@
 // terms) with FP32 accumulation (v_mfma_f32_16x16x32_bf16).
 //
 // It computes acc += A * B^T for a BM x BN block from panel tiles that were
-// converted once per step into BF16 parts (k_split_panel), staged through
-// LDS with double buffering like the FP64 core in mfma.hpp.
+// converted once per step into BF16 parts (k_split_panel), copied into a
+// three-stage LDS ring with direct-to-LDS loads.
 // Result layout (measured on gfx950 with an in-bounds single-wave probe):
 // lane l, element v holds D[4*(l/16)+v][l%16]; operand layout: lane l holds
 // A[l%16][8*(l/16)+e] and B^T[l%16][8*(l/16)+e], e = 0..7.
@@ -102,149 +102,14 @@ __device__ __forceinline__ void split_pair(float a, float b, uint32_t& hi, uint3
   }
 }
 
-// ---------------------------------------------------- pre-split panels --
-// Panel tiles converted once per step into BF16 hi (and lo) parts, laid out
-// per tile as [k-group g][row r][8 k values] (g = column / 8): one 16-byte
-// load is eight k values of one row, consecutive rows are consecutive in
-// memory, and consecutive lanes store consecutive LDS rows (no conflicts).
-// A part of tile (row block at ro) is at base + ro * 8 with group stride
-// ts * 8 elements.
-template <class C, int NSPLIT>
-struct Bf16pLds {
-  static_assert(C::BK % 32 == 0, "BK must be a multiple of the MFMA depth");
-  static constexpr int LDK = C::BK + 8;
-  static constexpr int A_ARR = C::BM * LDK, B_ARR = C::BN * LDK;
-  static constexpr int NARR = NSPLIT >= 3 ? 2 : 1;
-  static constexpr int STAGE = NARR * (A_ARR + B_ARR);
-  static constexpr int HALVES = 2 * STAGE;
-  static constexpr int G = C::BK / 8;                          // k-groups per slice
-  static constexpr int UNITS = NARR * G * (C::BM + C::BN);     // 16-byte pieces per slice
-  static constexpr int SLOTS = (UNITS + C::NT - 1) / C::NT;
-};
-
-__device__ __forceinline__ uint4 ld_global16(const uint16_t* p) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  typedef __attribute__((address_space(1))) const uint4 guint4;
-  return *reinterpret_cast<guint4*>(reinterpret_cast<uintptr_t>(p));
-#else
-  return *reinterpret_cast<const uint4*>(p);
-#endif
-}
-
-// acc += A * B^T from pre-split operands: Ah/Al (rows of A, hi/lo parts;
-// lo unused for NSPLIT = 1) and Bh/Bl, group stride gs rows.  A and B come
-// from the same panel buffer, so the caller subtracts the result.
-template <class C, int NSPLIT>
-__device__ __forceinline__ void gemm_nt_bf16p(fx4 (&acc)[C::FM][C::FN], const uint16_t* __restrict__ Ah,
-                                                  const uint16_t* __restrict__ Al, const uint16_t* __restrict__ Bh,
-                                                  const uint16_t* __restrict__ Bl, long gs, int K, uint16_t* lds) {
-  using L = Bf16pLds<C, NSPLIT>;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm = wave / C::WGN;
-  const int wn = wave - wm * C::WGN;
-  uint4 reg[L::SLOTS];
-  // unit u: array a (0: A hi, 1: B hi, 2: A lo, 3: B lo), group g, row r;
-  // consecutive u are consecutive rows of one (array, group).
-  auto locate = [&](int u, int& arr, int& g, int& r) {
-    const int per_a = L::G * C::BM, per_b = L::G * C::BN;
-    const int pair = u / (per_a + per_b);  // 0: hi, 1: lo
-    int w = u - pair * (per_a + per_b);
-    if (w < per_a) {
-      arr = 2 * pair;
-      g = w / C::BM;
-      r = w - g * C::BM;
-    } else {
-      w -= per_a;
-      arr = 2 * pair + 1;
-      g = w / C::BN;
-      r = w - g * C::BN;
-    }
-  };
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int t = 0; t < L::SLOTS; ++t) {
-      const int u = tid + t * C::NT;
-      if (u < L::UNITS) {
-        int arr, g, r;
-        locate(u, arr, g, r);
-        const uint16_t* base = arr == 0 ? Ah : (arr == 1 ? Bh : (arr == 2 ? Al : Bl));
-        reg[t] = ld_global16(base + ((long)(k0 / 8 + g) * gs + r) * 8);
-      }
-    }
-  };
-  auto sstore = [&](int buf) {
-    uint16_t* st = lds + buf * L::STAGE;
-#pragma unroll
-    for (int t = 0; t < L::SLOTS; ++t) {
-      const int u = tid + t * C::NT;
-      if (u < L::UNITS) {
-        int arr, g, r;
-        locate(u, arr, g, r);
-        const int off = (arr & 1) ? L::A_ARR : 0;
-        const int lo = (arr >= 2) ? (L::A_ARR + L::B_ARR) : 0;
-        *reinterpret_cast<uint4*>(st + lo + off + r * L::LDK + 8 * g) = reg[t];
-      }
-    }
-  };
-  auto compute = [&](int buf) {
-    const uint16_t* st = lds + buf * L::STAGE;
-#pragma unroll
-    for (int kk = 0; kk < C::BK; kk += 32) {
-      const int kcol = kk + 8 * (lane >> 4);
-      bf16x8 ah[C::FM], bh[C::FN];
-#pragma unroll
-      for (int i = 0; i < C::FM; ++i)
-        ah[i] = *reinterpret_cast<const bf16x8*>(st + (wm * C::WM + i * 16 + (lane & 15)) * L::LDK + kcol);
-#pragma unroll
-      for (int j = 0; j < C::FN; ++j)
-        bh[j] = *reinterpret_cast<const bf16x8*>(st + L::A_ARR + (wn * C::WN + j * 16 + (lane & 15)) * L::LDK +
-                                                 kcol);
-      if (NSPLIT >= 3) {
-        bf16x8 al[C::FM], bl[C::FN];
-        const uint16_t* sl = st + L::A_ARR + L::B_ARR;
-#pragma unroll
-        for (int i = 0; i < C::FM; ++i)
-          al[i] = *reinterpret_cast<const bf16x8*>(sl + (wm * C::WM + i * 16 + (lane & 15)) * L::LDK + kcol);
-#pragma unroll
-        for (int j = 0; j < C::FN; ++j)
-          bl[j] = *reinterpret_cast<const bf16x8*>(sl + L::A_ARR + (wn * C::WN + j * 16 + (lane & 15)) * L::LDK +
-                                                   kcol);
-#pragma unroll
-        for (int i = 0; i < C::FM; ++i)
-#pragma unroll
-          for (int j = 0; j < C::FN; ++j) {
-            // smallest terms first
-            if (NSPLIT == 4)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bl[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-          }
-      }
-#pragma unroll
-      for (int i = 0; i < C::FM; ++i)
-#pragma unroll
-        for (int j = 0; j < C::FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-    }
-  };
-  const int nk = K / C::BK;
-  gload(0);
-  sstore(0);
-  __syncthreads();
-  for (int kc = 0; kc < nk; ++kc) {
-    const int buf = kc & 1;
-    if (kc + 1 < nk) gload((kc + 1) * C::BK);
-    compute(buf);
-    __builtin_amdgcn_sched_barrier(0);
-    if (kc + 1 < nk) sstore(buf ^ 1);
-    __syncthreads();
-  }
-}
-
 // ------------------------------------------- direct-to-LDS operand loads --
-// The same product from the same split panels, with the operands copied by
+// Panel tiles are converted once per step into BF16 hi (and lo) parts (see
+// k_split_panel), laid out per tile as [k-group g][row r][8 k values]
+// (g = column / 8): one 16-byte load is eight k values of one row, and
+// consecutive rows are consecutive in memory.  The lo parts of a tile follow
+// its hi parts.
+//
+// The product copies the operands with
 // global_load_lds_dwordx4 (each wave-wide instruction writes 64 lanes x 16
 // bytes contiguously into LDS, no registers involved) into a three-stage
 // ring.  The LDS layout of a stage is the panel's own [k-group][row][8]:
@@ -279,7 +144,8 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
 }
 
-// acc += A * B^T as gemm_nt_bf16p, for K a multiple of BK.  The ring holds
+// acc += A * B^T (A and B rows from the split panels, so the caller
+// subtracts the result), for K a multiple of BK.  The ring holds
 // slices kc, kc+1, kc+2: at step kc each wave waits for its own copies of
 // slice kc, a barrier makes every wave's copies visible and ends every read
 // of slice kc-1, whose buffer then receives slice kc+2.  The barrier is a
@@ -339,21 +205,31 @@ __device__ __forceinline__ void gemm_nt_bf16d(fx4 (&acc)[C::FM][C::FN], const ui
 #pragma unroll
         for (int j = 0; j < C::FN; ++j)
           bl[j] = *reinterpret_cast<const bf16x8*>(sl + L::A_ARR + (gq * C::BN + wn * C::WN + j * 16 + (lane & 15)) * 8);
+        // Each 32-deep chunk is summed by its own MFMA chain from zero
+        // (smallest terms first) and added to the running sum with one
+        // round-to-nearest FP32 add: the MFMA's internal accumulation is
+        // not correctly rounded (about 1 ulp of error per instruction,
+        // measured on gfx950), and chaining all 3 K / 32 instructions into
+        // one accumulator lets that error build up at the running sum's
+        // magnitude.
 #pragma unroll
         for (int i = 0; i < C::FM; ++i)
 #pragma unroll
           for (int j = 0; j < C::FN; ++j) {
-            if (NSPLIT == 4)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bl[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            fx4 t = {0.f, 0.f, 0.f, 0.f};
+            if (NSPLIT == 4) t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bl[j], t, 0, 0, 0);
+            t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], t, 0, 0, 0);
+            t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], t, 0, 0, 0);
+            t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], t, 0, 0, 0);
+            acc[i][j] += t;
           }
+      } else {
+#pragma unroll
+        for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+          for (int j = 0; j < C::FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
       }
-#pragma unroll
-      for (int i = 0; i < C::FM; ++i)
-#pragma unroll
-        for (int j = 0; j < C::FN; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
     }
   };
   const int nk = K / C::BK;

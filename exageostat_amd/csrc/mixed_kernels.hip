@@ -91,11 +91,6 @@ void launch_split_panel(const TileView& v, int k, uint16_t* split, hipStream_t s
 
 // --------------------------------------------------------- class updates --
 // One block of C(m,n) -= A(m,k) B(n,k)^T, C of precision class CLS.
-// The BF16 classes copy their operands with direct-to-LDS loads into a
-// three-stage ring (gemm_nt_bf16d); false: register-staged double buffer
-// (gemm_nt_bf16p).
-constexpr bool kDirectLds = true;
-
 template <int CLS, int BS>
 struct ClassCfg;
 // FP64 class: the FP64 core.
@@ -111,7 +106,7 @@ struct ClassCfg<kF32, BS> {
   using C = typename std::conditional<BS == 192, LpCfg<192, 192, 2, 4, 32>,
             typename std::conditional<BS == 128, LpCfg<128, 128, 2, 2, 32>, LpCfg<64, 64, 2, 2, 32>>::type>::type;
   static constexpr int NSPLIT = 3;
-  static constexpr int LDS_BYTES = (kDirectLds ? Bf16dLds<C, NSPLIT>::HALVES : Bf16pLds<C, NSPLIT>::HALVES) * 2;
+  static constexpr int LDS_BYTES = Bf16dLds<C, NSPLIT>::HALVES * 2;
 };
 // FP32 class with the lo*lo product as well (four split-BF16 products); a
 // kernel variant of kF32, not a storage class.
@@ -120,7 +115,7 @@ template <int BS>
 struct ClassCfg<kF32Q, BS> {
   using C = typename ClassCfg<kF32, BS>::C;
   static constexpr int NSPLIT = 4;
-  static constexpr int LDS_BYTES = (kDirectLds ? Bf16dLds<C, NSPLIT>::HALVES : Bf16pLds<C, NSPLIT>::HALVES) * 2;
+  static constexpr int LDS_BYTES = Bf16dLds<C, NSPLIT>::HALVES * 2;
 };
 // Far class: FP32 storage, single BF16 MFMA operands.
 template <int BS>
@@ -128,7 +123,7 @@ struct ClassCfg<kF32B, BS> {
   using C = typename std::conditional<BS == 192, LpCfg<192, 192, 2, 4, 64>,
             typename std::conditional<BS == 128, LpCfg<128, 128, 2, 2, 64>, LpCfg<64, 64, 2, 2, 64>>::type>::type;
   static constexpr int NSPLIT = 1;
-  static constexpr int LDS_BYTES = (kDirectLds ? Bf16dLds<C, NSPLIT>::HALVES : Bf16pLds<C, NSPLIT>::HALVES) * 2;
+  static constexpr int LDS_BYTES = Bf16dLds<C, NSPLIT>::HALVES * 2;
 };
 
 // FP64 class: one block of C(m,n) -= A(m,k) B(n,k)^T on the FP64 matrix
@@ -172,11 +167,7 @@ __device__ __forceinline__ void class_update(const TileView& v, int k, int m, in
     const uint16_t* Bh = split + 2 * (long)n * tt + (long)bj * C::BN * 8;
     fx4 acc[C::FM][C::FN];
     zero_acc_lp<C>(acc);
-    if constexpr (kDirectLds)
-      gemm_nt_bf16d<C, ClassCfg<CLS, BS>::NSPLIT>(acc, Ah, Bh, tt, v.ts, v.ts, reinterpret_cast<uint16_t*>(lds));
-    else
-      gemm_nt_bf16p<C, ClassCfg<CLS, BS>::NSPLIT>(acc, Ah, Ah + tt, Bh, Bh + tt, v.ts, v.ts,
-                                                  reinterpret_cast<uint16_t*>(lds));
+    gemm_nt_bf16d<C, ClassCfg<CLS, BS>::NSPLIT>(acc, Ah, Bh, tt, v.ts, v.ts, reinterpret_cast<uint16_t*>(lds));
     acc_add_lp<C, float>(acc, static_cast<float*>(v.ptr[tc]) + co, v.ts, -1.f);
   }
 }
