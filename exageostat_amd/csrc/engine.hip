@@ -395,16 +395,25 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
       profiled("tile_update", update_flops(A, k, lo, hi), 0.0, s, [&] { launch_tile_update(v, k, lo, hi, s); });
       return;
     }
-    static const char* names[4] = {"tile_update_f64", "tile_update_f32", "", "tile_update_bf16"};
+    // One fused launch for the three classes; each class's share of the
+    // algorithmic flops is counted under its own name as well.
+    static const char* names[4] = {"mixed_flops_f64", "mixed_flops_f32", "", "mixed_flops_bf16"};
+    FusedSpans sp;
+    double fl = 0.0;
+    int i = 0;
     for (int c : {(int)kF64, (int)kF32, (int)kF32B}) {
       const long beg = lists->off[c][lo];
       const long end = lists->off[c][std::min(hi + 1, mt)];
-      if (end <= beg) continue;
-      const double fl = (lists->flops[c][end] - lists->flops[c][beg]) * rows(k);
-      profiled(names[c], fl, 0.0, s, [&] {
-        launch_update_class(c, v, k, lists->d_pairs[c] + beg, end - beg, lists->split[k & 1], lists->lolo, s);
-      });
+      sp.pairs[i] = (lists->d_pairs[c] && end > beg) ? lists->d_pairs[c] + beg : nullptr;
+      sp.nblocks[i] = end > beg ? end - beg : 0;
+      const double f = (lists->flops[c][end] - lists->flops[c][beg]) * rows(k);
+      if (f > 0) Profiler::get().count(names[c], f, 0.0);
+      fl += f;
+      ++i;
     }
+    if (sp.nblocks[0] + sp.nblocks[1] + sp.nblocks[2] == 0) return;
+    profiled("tile_update_mixed", fl, 0.0, s,
+             [&] { launch_update_fused(v, k, sp, lists->split[k & 1], lists->lolo, s); });
   };
   // s1 carries the panel factorisations (small kernels on the CUs the
   // persistent update leaves free); s2 carries the trailing updates: first

@@ -83,12 +83,18 @@ void launch_convert_mixed(const TileView& src, const TileView& dst, long ntiles,
 // BF16 hi/lo parts of the panel tiles (m, k), m > k, for the BF16 update
 // classes: tile m at split + 2 m ts^2 (mt tiles of room).
 void launch_split_panel(const TileView& v, int k, uint16_t* split, hipStream_t s);
-// Trailing update of step k for the tile pairs (packed m << 16 | n) of one
-// precision class (kF64, kF32 or kF32B) of the updated tile; the BF16
-// classes read their operands from the split panel of step k.  lolo: the
-// kF32 class adds the lo*lo product (four split-BF16 products).
-void launch_update_class(int cls, const TileView& v, int k, const int* pairs, long npairs, const uint16_t* split,
-                         bool lolo, hipStream_t s);
+// Trailing update of step k over the tile pairs (packed m << 16 | n) of the
+// three precision classes of the updated tile, in one launch: pairs[0] /
+// nblocks[0] FP64 tiles (FP64 MFMA), [1] FP32 tiles (split-BF16 MFMA, with
+// the lo*lo product when lolo), [2] far tiles (BF16 MFMA).  nblocks counts
+// tile pairs; the BF16 classes read their operands from the split panel of
+// step k.
+struct FusedSpans {
+  const int* pairs[3];
+  long nblocks[3];
+};
+void launch_update_fused(const TileView& v, int k, const FusedSpans& sp, const uint16_t* split, bool lolo,
+                         hipStream_t s);
 // out = a*x + b*y (n entries).
 void launch_axpby(double a, const double* x, double b, const double* y, double* out, long n, hipStream_t s);
 // y = Sigma x for a lower-packed FP64 matrix with full diagonal tiles, each

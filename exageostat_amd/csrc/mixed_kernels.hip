@@ -172,50 +172,76 @@ __device__ __forceinline__ void class_update(const TileView& v, int k, int m, in
   }
 }
 
-// Persistent loop over nblocks blocks of the tile pairs pairs[] (packed
-// m << 16 | n, q*q blocks each): work-group w runs on XCD w % nx and the
-// XCDs take contiguous slices of the block list, as in the FP64 update.
-template <int BS, class F>
-__device__ __forceinline__ void for_each_block(const TileView& v, const int* pairs, long nblocks, int nx, F&& f) {
+// The trailing update of one step over all three precision classes in one
+// persistent launch.  XCD x takes slice x of the FP64 class's block list,
+// then slice x of the FP32 list, then of the far list (the slowest blocks
+// first; contiguous slices keep the A and B panels hot in the XCD's L2), and
+// its work-groups stride through that sequence.  One launch instead of one
+// per class removes two launch tails per update.  Blocks are q x q per tile
+// pair (packed m << 16 | n); the strictly upper blocks of diagonal tiles
+// are skipped.
+template <int BS, int F32CLS>
+__global__ void __launch_bounds__((ClassCfg<kF64, BS>::C::NT))
+    k_update_fused(TileView v, int k, FusedSpans sp, int nx, const uint16_t* split) {
+  static_assert(ClassCfg<kF64, BS>::C::NT == ClassCfg<F32CLS, BS>::C::NT &&
+                    ClassCfg<kF64, BS>::C::NT == ClassCfg<kF32B, BS>::C::NT,
+                "the classes share one work-group shape");
+  constexpr int LB = ClassCfg<F32CLS, BS>::LDS_BYTES > ClassCfg<kF64, BS>::LDS_BYTES
+                         ? ClassCfg<F32CLS, BS>::LDS_BYTES
+                         : ClassCfg<kF64, BS>::LDS_BYTES;
+  __shared__ __attribute__((aligned(16))) unsigned char lds[LB];
   const int q = v.ts / BS;
   const int w = blockIdx.x;
   const int x = w % nx;
   const int gx = gridDim.x / nx;
-  const long per = (nblocks + nx - 1) / nx;
-  const long start = (long)x * per;
-  const long end = start + per < nblocks ? start + per : nblocks;
-  for (long b = start + w / nx; b < end; b += gx) {
+  long st[3], cnt[3], tot = 0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const long per = (sp.nblocks[c] + nx - 1) / nx;
+    st[c] = (long)x * per;
+    const long e = st[c] + per < sp.nblocks[c] ? st[c] + per : sp.nblocks[c];
+    cnt[c] = e > st[c] ? e - st[c] : 0;
+    tot += cnt[c];
+  }
+  for (long i = w / nx; i < tot; i += gx) {
+    int c = 0;
+    long j = i;
+    if (j >= cnt[0]) {
+      j -= cnt[0];
+      c = 1;
+      if (j >= cnt[1]) {
+        j -= cnt[1];
+        c = 2;
+      }
+    }
+    const long b = st[c] + j;
     const long p = b / (q * q);
     const int rem = (int)(b - p * (q * q));
     const int bi = rem % q;
     const int bj = rem / q;
-    const int pk = pairs[p];
+    const int pk = sp.pairs[c][p];
     const int m = pk >> 16;
     const int n = pk & 0xFFFF;
-    if (m == n && bi < bj) continue;  // strictly upper block of a diagonal tile
-    f(m, n, bi, bj);
+    if (m == n && bi < bj) continue;
+    if (c == 0) class_update<kF64, BS>(v, k, m, n, bi, bj, split, lds);
+    else if (c == 1) class_update<F32CLS, BS>(v, k, m, n, bi, bj, split, lds);
+    else class_update<kF32B, BS>(v, k, m, n, bi, bj, split, lds);
   }
 }
 
-// Update over the pairs of one precision class CLS.
-template <int CLS, int BS>
-__global__ void __launch_bounds__((ClassCfg<CLS, BS>::C::NT))
-    k_update_class(TileView v, int k, const int* pairs, long nblocks, int nx, const uint16_t* split) {
-  __shared__ __attribute__((aligned(16))) unsigned char lds[ClassCfg<CLS, BS>::LDS_BYTES];
-  for_each_block<BS>(v, pairs, nblocks, nx,
-                     [&](int m, int n, int bi, int bj) { class_update<CLS, BS>(v, k, m, n, bi, bj, split, lds); });
-}
-
-template <int CLS, int BS>
-static int class_grid_cap() {
+// Work-groups of the fused update: one per CU less the CUs left to the
+// panel factorisation on the other stream (32 by default, 4 per XCD; with
+// fewer the diagonal factorisation starves), times the occupancy.
+template <int BS, int F32CLS>
+static int fused_grid_cap() {
   static int cap = 0;
   if (cap) return cap;
   int dev = 0, cus = 0, occ = 1;
   EXA_HIP(hipGetDevice(&dev));
   EXA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  EXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_update_class<CLS, BS>,
-                                                       ClassCfg<CLS, BS>::C::NT, 0));
-  int reserve = 16;
+  EXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_update_fused<BS, F32CLS>,
+                                                       ClassCfg<kF64, BS>::C::NT, 0));
+  int reserve = 32;
   if (const char* e = getenv("EXAGEOSTAT_RESERVE_CUS")) reserve = atoi(e);
   if (reserve < 0 || reserve >= cus) reserve = 0;
   cap = (cus - reserve) * (occ > 0 ? occ : 1);
@@ -224,35 +250,37 @@ static int class_grid_cap() {
   return cap;
 }
 
-template <int CLS, int BS>
-static void launch_class_bs(const TileView& v, int k, const int* pairs, long npairs, const uint16_t* split,
-                            hipStream_t s) {
+template <int BS, int F32CLS>
+static void launch_fused_bs(const TileView& v, int k, const FusedSpans& sp0, const uint16_t* split, hipStream_t s) {
   const long q = v.ts / BS;
-  const long blocks = npairs * q * q;
-  const int cap = class_grid_cap<CLS, BS>();
-  const int grid = blocks < cap ? (int)blocks : cap;
+  FusedSpans sp = sp0;
+  long total = 0;
+  for (int c = 0; c < 3; ++c) {
+    sp.nblocks[c] = sp0.nblocks[c] * q * q;  // pairs -> blocks
+    total += sp.nblocks[c];
+  }
+  if (total <= 0) return;
+  const int cap = fused_grid_cap<BS, F32CLS>();
+  const int grid = total < cap ? (int)total : cap;
   const int nx = (grid % 8 == 0) ? 8 : 1;
-  hipLaunchKernelGGL((k_update_class<CLS, BS>), dim3((unsigned)grid), dim3(ClassCfg<CLS, BS>::C::NT), 0, s, v, k,
-                     pairs, blocks, nx, split);
+  hipLaunchKernelGGL((k_update_fused<BS, F32CLS>), dim3((unsigned)grid), dim3(ClassCfg<kF64, BS>::C::NT), 0, s, v,
+                     k, sp, nx, split);
   EXA_HIP(hipGetLastError());
 }
 
-template <int CLS>
-static void launch_class(const TileView& v, int k, const int* pairs, long npairs, const uint16_t* split,
-                         hipStream_t s) {
-  if (v.ts % 192 == 0) launch_class_bs<CLS, 192>(v, k, pairs, npairs, split, s);
-  else if (v.ts % 128 == 0) launch_class_bs<CLS, 128>(v, k, pairs, npairs, split, s);
-  else launch_class_bs<CLS, 64>(v, k, pairs, npairs, split, s);
+template <int F32CLS>
+static void launch_fused(const TileView& v, int k, const FusedSpans& sp, const uint16_t* split, hipStream_t s) {
+  if (v.ts % 192 == 0) launch_fused_bs<192, F32CLS>(v, k, sp, split, s);
+  else if (v.ts % 128 == 0) launch_fused_bs<128, F32CLS>(v, k, sp, split, s);
+  else launch_fused_bs<64, F32CLS>(v, k, sp, split, s);
 }
 
-void launch_update_class(int cls, const TileView& v, int k, const int* pairs, long npairs, const uint16_t* split,
-                         bool lolo, hipStream_t s) {
-  if (npairs <= 0) return;
-  if (cls != kF64 && split == nullptr) throw std::invalid_argument("BF16 update classes need the split panel");
-  if (cls == kF64) launch_class<kF64>(v, k, pairs, npairs, split, s);
-  else if (cls == kF32 && lolo) launch_class<kF32Q>(v, k, pairs, npairs, split, s);
-  else if (cls == kF32) launch_class<kF32>(v, k, pairs, npairs, split, s);
-  else launch_class<kF32B>(v, k, pairs, npairs, split, s);
+void launch_update_fused(const TileView& v, int k, const FusedSpans& sp, const uint16_t* split, bool lolo,
+                         hipStream_t s) {
+  if ((sp.nblocks[1] > 0 || sp.nblocks[2] > 0) && split == nullptr)
+    throw std::invalid_argument("BF16 update classes need the split panel");
+  if (lolo) launch_fused<kF32Q>(v, k, sp, split, s);
+  else launch_fused<kF32>(v, k, sp, split, s);
 }
 
 // ------------------------------------------------ symmetric product (IR) --

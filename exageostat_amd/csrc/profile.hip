@@ -34,6 +34,14 @@ int Profiler::begin(const char* name, double flops, double bytes, hipStream_t s)
 
 void Profiler::end(int handle, hipStream_t s) { EXA_HIP(hipEventRecord(event(pending_[handle].ev_b), s)); }
 
+void Profiler::count(const char* name, double flops, double bytes) {
+  if (!enabled_) return;
+  KernelTotals& t = totals_[name];
+  t.launches += 1;
+  t.flops += flops;
+  t.bytes += bytes;
+}
+
 void Profiler::collect() {
   for (const Pending& p : pending_) {
     EXA_HIP(hipEventSynchronize(event(p.ev_b)));

@@ -26,6 +26,9 @@ class Profiler {
   // Returns a handle for end(); records the start event on s.
   int begin(const char* name, double flops, double bytes, hipStream_t s);
   void end(int handle, hipStream_t s);
+  // Adds flops and bytes to a group without timing (a share of work done
+  // inside another group's launch).
+  void count(const char* name, double flops, double bytes);
   // Resolves pending events (requires them to have completed).
   void collect();
   void reset();
