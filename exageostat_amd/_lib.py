@@ -108,6 +108,10 @@ _SIGNATURES = {
     "exageostat_gpu_profile_enable": (None, [ctypes.c_int]),
     "exageostat_gpu_profile_reset": (None, []),
     "exageostat_mixed_prec_map": (ctypes.c_long, [_P(MLEData), _P(ctypes.c_ubyte), ctypes.c_long]),
+    "exageostat_fisher": (ctypes.c_int, [_P(MLEData), _P(ctypes.c_double), _P(ctypes.c_double),
+                                         _P(ctypes.c_double)]),
+    "EXAGEOSTAT_Fisher_Tile": (_P(ctypes.c_double), [_P(MLEData), ctypes.c_int, _P(ctypes.c_double), ctypes.c_int,
+                                                     ctypes.c_int, ctypes.c_int]),
     "exageostat_gpu_profile_get": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_long), c_double_p, c_double_p,
                                                   c_double_p]),
     "exageostat_gpu_last_status": (ctypes.c_int, []),

@@ -25,6 +25,12 @@ namespace exa {
 
 struct AdjointState {
   DeviceVector alpha;
+  // Fisher information: dSigma_p, Z dSigma_p Z (lower tiles), Z dSigma_p
+  // (full tile matrix) and a zero vector.
+  TileMatrix D, M;
+  double* Y = nullptr;
+  long y_tiles = 0;
+  DeviceVector zero;
   double* P = nullptr;  // mt tiles: L_jk X_kk of the current column
   double* Q = nullptr;  // mt tiles: split-k terms of the diagonal tile
   long tiles = 0;
@@ -33,6 +39,7 @@ struct AdjointState {
   double* sums = nullptr;  // 8 * kGroups
   KnuOrder<Dual>* ko = nullptr;  // device: K_nu constants of the bivariate kernel
   ~AdjointState() {
+    if (Y) (void)hipFree(Y);
     if (ko) (void)hipFree(ko);
     if (P) (void)hipFree(P);
     if (Q) (void)hipFree(Q);
@@ -67,6 +74,57 @@ void tile_potri_inplace(TileMatrix& A, const CholWorkspace& ws, double* P, doubl
   }
 }
 
+// Scratch of the Takahashi sweep and of the dSigma reductions, sized for
+// ctx.C and kept across calls.
+static AdjointState& adjoint_buffers(Context& ctx) {
+  if (!ctx.adjoint) ctx.adjoint = new AdjointState();
+  AdjointState& st = *ctx.adjoint;
+  TileMatrix& A = ctx.C;
+  const long tile_elems = (long)A.ts * A.ts;
+  if (st.tiles < A.mt) {
+    if (st.P) (void)hipFree(st.P);
+    if (st.Q) (void)hipFree(st.Q);
+    st.P = st.Q = nullptr;
+    EXA_HIP(hipMalloc(&st.P, sizeof(double) * tile_elems * A.mt));
+    EXA_HIP(hipMalloc(&st.Q, sizeof(double) * tile_elems * A.mt));
+    st.tiles = A.mt;
+  }
+  const long nb = grad_partial_blocks(A.view());
+  if (st.partial_blocks < nb) {
+    if (st.partial) (void)hipFree(st.partial);
+    st.partial = nullptr;
+    EXA_HIP(hipMalloc(&st.partial, sizeof(double) * 8 * nb));
+    st.partial_blocks = nb;
+  }
+  if (!st.sums) EXA_HIP(hipMalloc(&st.sums, sizeof(double) * 8 * kGroups));
+  return st;
+}
+
+static MaternGradParams matern_grad_params(const Context& ctx, KernelId kid, const double* theta, int metric) {
+  MaternGradParams G;
+  G.x = ctx.l1.dx;
+  G.y = ctx.l1.dy;
+  G.n = ctx.N;
+  G.metric = metric;
+  G.nugget = kid == kNugget ? 1 : 0;
+  G.sigma2 = theta[0];
+  G.beta = theta[1];
+  G.con = host_matern_con(theta[2]);
+  knu_prepare(theta[2], G.ko);
+  return G;
+}
+
+// Sums the per-group partials of a reduction launch: out[p], p < nsum.
+static void reduce_sums(AdjointState& st, int nsum, hipStream_t s, double* out) {
+  std::vector<double> h((size_t)nsum * kGroups);
+  EXA_HIP(hipMemcpyAsync(h.data(), st.sums, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+  EXA_HIP(hipStreamSynchronize(s));
+  for (int p = 0; p < nsum; ++p) {
+    out[p] = 0.0;
+    for (int g = 0; g < kGroups; ++g) out[p] += h[(size_t)p * kGroups + g];
+  }
+}
+
 double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric, double* grad,
                            double* logdet_out, double* dotp_out, int* info) {
   if (kid != kUnivariate && kid != kNugget && kid != kBivariate) throw std::invalid_argument("unsupported kernel");
@@ -83,27 +141,9 @@ double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int 
     for (int p = 0; p < np; ++p) grad[p] = std::nan("");
     return ll;
   }
-  if (!ctx.adjoint) ctx.adjoint = new AdjointState();
-  AdjointState& st = *ctx.adjoint;
+  AdjointState& st = adjoint_buffers(ctx);
   TileMatrix& A = ctx.C;
-  const long tile_elems = (long)A.ts * A.ts;
-  if (st.tiles < A.mt) {
-    if (st.P) (void)hipFree(st.P);
-    if (st.Q) (void)hipFree(st.Q);
-    st.P = st.Q = nullptr;
-    EXA_HIP(hipMalloc(&st.P, sizeof(double) * tile_elems * A.mt));
-    EXA_HIP(hipMalloc(&st.Q, sizeof(double) * tile_elems * A.mt));
-    st.tiles = A.mt;
-  }
   const TileView v = A.view();
-  const long nb = grad_partial_blocks(v);
-  if (st.partial_blocks < nb) {
-    if (st.partial) (void)hipFree(st.partial);
-    st.partial = nullptr;
-    EXA_HIP(hipMalloc(&st.partial, sizeof(double) * 8 * nb));
-    st.partial_blocks = nb;
-  }
-  if (!st.sums) EXA_HIP(hipMalloc(&st.sums, sizeof(double) * 8 * kGroups));
   hipStream_t s = ctx.s1;
 
   // alpha = L^-T (L^-1 z); ctx.work holds L^-1 z after loglik.
@@ -142,32 +182,68 @@ double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int 
     profiled("grad_reduce", 0.0, 4.0 * (double)ctx.N * (ctx.N + 1.0), s,
              [&] { launch_grad_bivariate(v, B, st.ko, st.alpha.d, st.partial, st.sums, kGroups, s); });
   } else {
-    MaternGradParams G;
-    G.x = ctx.l1.dx;
-    G.y = ctx.l1.dy;
-    G.n = ctx.N;
-    G.metric = metric;
-    G.nugget = kid == kNugget ? 1 : 0;
-    G.sigma2 = theta[0];
-    G.beta = theta[1];
-    G.con = host_matern_con(theta[2]);
-    knu_prepare(theta[2], G.ko);
+    const MaternGradParams G = matern_grad_params(ctx, kid, theta, metric);
     profiled("grad_reduce", 0.0, 4.0 * (double)ctx.N * (ctx.N + 1.0), s,
              [&] { launch_grad_matern(v, G, st.alpha.d, st.partial, st.sums, kGroups, s); });
   }
-  std::vector<double> h((size_t)nsum * kGroups);
-  EXA_HIP(hipMemcpyAsync(h.data(), st.sums, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
-  EXA_HIP(hipStreamSynchronize(s));
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  reduce_sums(st, nsum, s, acc);
   if (Profiler::get().enabled()) Profiler::get().collect();
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int p = 0; p < nsum; ++p)
-    for (int g = 0; g < kGroups; ++g) acc[p] += h[(size_t)p * kGroups + g];
   for (int p = 0; p < np; ++p) grad[p] = -0.5 * acc[p];
   const double t1 = wall_seconds();
   ctx.last = lt;
   ctx.last.grad = t1 - t0 - lt.total;
   ctx.last.total = t1 - t0;
   return ll;
+}
+
+// Fisher information (reference: EXAGEOSTAT_Fisher_Tile, exageostat_exact/
+// src/compute/MLE_dexact.c:1777-1934, which forms Sigma^-1 dSigma_j by two
+// triangular solves per parameter on stored derivative matrices and the
+// trace of each product).  Here Z = Sigma^-1 comes from the Takahashi sweep,
+// M_j = Z D_j Z from two tile products per parameter, and <M_j, D_k> for
+// every k from one pass of the fused derivative reduction, which
+// regenerates D_k entry by entry.
+int fisher_information(Context& ctx, KernelId kid, const double* theta, int metric, double* F) {
+  if (kid != kUnivariate && kid != kNugget)
+    throw std::invalid_argument("Fisher information: univariate Matern kernels only");
+  const int np = kernel_num_params(kid);
+  int inf = 0;
+  double logdet = 0.0, dotp = 0.0;
+  ctx.loglik(kid, theta, metric, &logdet, &dotp, &inf);
+  if (inf != 0) return inf;
+  AdjointState& st = adjoint_buffers(ctx);
+  TileMatrix& A = ctx.C;
+  hipStream_t s = ctx.s1;
+  if (!st.D.allocated() || st.D.mt != A.mt || st.D.ts != A.ts) {
+    st.D.release();
+    st.M.release();
+    st.D.allocate(A.m, A.n, A.ts, true);
+    st.M.allocate(A.m, A.n, A.ts, true);
+  }
+  if (st.y_tiles < (long)A.mt * A.mt) {
+    if (st.Y) (void)hipFree(st.Y);
+    st.Y = nullptr;
+    EXA_HIP(hipMalloc(&st.Y, sizeof(double) * (size_t)A.ts * A.ts * A.mt * A.mt));
+    st.y_tiles = (long)A.mt * A.mt;
+  }
+  st.zero.allocate(ctx.N, ctx.work.padded);
+  tile_potri_inplace(A, ctx.ws, st.P, st.Q, s);
+  const MaternGradParams G = matern_grad_params(ctx, kid, theta, metric);
+  const double n = (double)ctx.N;
+  for (int j = 0; j < np; ++j) {
+    profiled("fisher_gen", 0.0, 4.0 * n * (n + 1.0), s, [&] { launch_gen_dmatern(st.D.view(), G, j, s); });
+    profiled("fisher_zdz", 3.0 * n * n * n, 0.0, s,
+             [&] { launch_fisher_zdz(A.view(), st.D.view(), st.Y, st.M.view(), s); });
+    profiled("fisher_reduce", 0.0, 4.0 * n * (n + 1.0), s,
+             [&] { launch_grad_matern(st.M.view(), G, st.zero.d, st.partial, st.sums, kGroups, s); });
+    double acc[8];
+    reduce_sums(st, 4, s, acc);
+    // row j from pass j for k >= j, as the reference fills its lower triangle
+    for (int k = j; k < np; ++k) F[j * np + k] = F[k * np + j] = 0.5 * acc[k];
+  }
+  if (Profiler::get().enabled()) Profiler::get().collect();
+  return 0;
 }
 
 }  // namespace exa

@@ -13,4 +13,10 @@ void tile_potri_inplace(TileMatrix& A, const CholWorkspace& ws, double* P, doubl
 double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric, double* grad,
                            double* logdet_out, double* dotp_out, int* info);
 
+// Fisher information F_jk = 1/2 tr(Sigma^-1 dSigma_j Sigma^-1 dSigma_k) of
+// the univariate Matern kernels at theta (np x np, row-major, symmetric).
+// Returns 0, or the first failing column + 1 when Sigma(theta) is not
+// positive definite.  Leaves Sigma^-1 in place of the factor.
+int fisher_information(Context& ctx, KernelId kid, const double* theta, int metric, double* F);
+
 }  // namespace exa

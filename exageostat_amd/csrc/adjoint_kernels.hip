@@ -284,6 +284,136 @@ __global__ void __launch_bounds__(256) k_grad_bivariate(TileView v, BivariateGra
   }
 }
 
+// ------------------------------------------------------ Fisher information --
+// F_jk = 1/2 tr(Z D_j Z D_k) with Z = Sigma^-1 and D_p = dSigma/dtheta_p is
+// evaluated as 1/2 <M_j, D_k>_F with M_j = Z D_j Z: D_j is generated into
+// lower tiles, Y = Z D_j (full tile matrix), M_j = Y Z (lower tiles), and
+// the gradient reduction over W = M_j (alpha = 0) gives <M_j, D_k> for
+// every k at once.
+
+// D_p into the lower tiles of v (diagonal tiles full); zero outside the
+// logical n x n matrix.  One work-group per 64 x 64 block, 64 x 4 threads.
+__global__ void __launch_bounds__(256) k_gen_dmatern(TileView v, MaternGradParams P, int p) {
+  const int q = v.ts / kNB;
+  const long b = blockIdx.x;
+  const long t = b / (q * q);
+  const int rem = (int)(b - t * (q * q));
+  const int bj = rem / q, bi = rem - (rem / q) * q;
+  int m, n;
+  tri_decode(t, m, n);
+  const int ts = v.ts;
+  double* D = tile_f64(v, m, n);
+  const int i = bi * kNB + threadIdx.x;
+  const long gi = (long)m * ts + i;
+  for (int jj = threadIdx.y; jj < kNB; jj += 4) {
+    const int j = bj * kNB + jj;
+    const long gj = (long)n * ts + j;
+    double val = 0.0;
+    if (gi < P.n && gj < P.n) {
+      const double d = grad_distance(P.x[gi], P.y[gi], P.x[gj], P.y[gj], P.metric);
+      if (d == 0.0) {
+        val = (p == 0 || (p == 3 && P.nugget)) ? 1.0 : 0.0;
+      } else if (p < 3) {
+        const double xx = d / P.beta;
+        Dual kn, kn1;
+        knu_pair(P.ko, xx, kn, kn1);
+        const Dual xnu = exp(P.ko.nu * ::log(xx));
+        const Dual corr = P.con * xnu * kn;
+        if (p == 0) val = corr.v;
+        else if (p == 1) val = P.sigma2 * P.con.v * (xnu.v * xx * kn1.v - 2.0 * P.ko.nu.v * xnu.v * kn.v) / P.beta;
+        else val = P.sigma2 * corr.d;
+      }
+    }
+    D[i + (long)j * ts] = val;
+  }
+}
+
+// Y(m,n) = sum_l Zs(m,l) Ds(l,n) over the l of one operand layout, for the
+// full tile matrix Y (tile (m,n) at Y + (n mt + m) ts^2); Zs and Ds are the
+// symmetric matrices held in the lower tiles of Z and D.  AK: l > m (A read
+// K-major from Z(l,m)); BK: l >= n (B read K-major from D(l,n)).  The launch
+// with AK = BK = false covers l < min(m+1, n) and writes every block; the
+// others accumulate.
+template <class C, bool AK, bool BK>
+__global__ void __launch_bounds__(C::NT) k_fisher_zd(TileView z, TileView d, double* Y) {
+  __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
+  const int qm = z.ts / C::BM, qn = z.ts / C::BN;
+  const long b = blockIdx.x;
+  const long tp = b / (qm * qn);
+  const int rem = (int)(b % (qm * qn));
+  const int bi = rem % qm, bj = rem / qm;
+  const int mt = z.mt;
+  const int m = (int)(tp % mt), n = (int)(tp / mt);
+  const long ts = z.ts;
+  int l0, l1;  // [l0, l1)
+  if (!AK && !BK) { l0 = 0; l1 = min(m + 1, n); }
+  else if (!AK && BK) { l0 = n; l1 = m + 1; }
+  else if (AK && BK) { l0 = max(m + 1, n); l1 = mt; }
+  else { l0 = m + 1; l1 = n; }
+  const bool first = !AK && !BK;
+  if (!first && l0 >= l1) return;
+  dx4 acc[C::FM][C::FN];
+  zero_acc<C>(acc);
+  for (int l = l0; l < l1; ++l) {
+    const double* A = AK ? tile_f64(z, l, m) + (long)bi * C::BM * ts : tile_f64(z, m, l) + bi * C::BM;
+    const double* B = BK ? tile_f64(d, l, n) + (long)bj * C::BN * ts : tile_f64(d, n, l) + bj * C::BN;
+    gemm_nt_acc<C, BK, double, double, AK>(acc, A, ts, B, ts, (int)ts, lds, 1.0);
+  }
+  acc_update<C>(acc, Y + ((long)n * mt + m) * ts * ts + bi * C::BM + (long)bj * C::BN * ts, ts, first ? 0.0 : 1.0);
+}
+
+// M(m,n) = sum_l Y(m,l) Zs(l,n) for the lower tiles m >= n of M.  BK: l >= n
+// (B read K-major from Z(l,n)); the launch with BK = false (l < n) writes
+// every block.
+template <class C, bool BK>
+__global__ void __launch_bounds__(C::NT) k_fisher_yz(const double* Y, TileView z, TileView mo) {
+  __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
+  const int qm = z.ts / C::BM, qn = z.ts / C::BN;
+  const long b = blockIdx.x;
+  const long tp = b / (qm * qn);
+  const int rem = (int)(b % (qm * qn));
+  const int bi = rem % qm, bj = rem / qm;
+  int m, n;
+  tri_decode(tp, m, n);
+  const int mt = z.mt;
+  const long ts = z.ts;
+  const int l0 = BK ? n : 0, l1 = BK ? mt : n;
+  if (BK && l0 >= l1) return;
+  dx4 acc[C::FM][C::FN];
+  zero_acc<C>(acc);
+  for (int l = l0; l < l1; ++l) {
+    const double* A = Y + ((long)l * mt + m) * ts * ts + bi * C::BM;
+    const double* B = BK ? tile_f64(z, l, n) + (long)bj * C::BN * ts : tile_f64(z, n, l) + bj * C::BN;
+    gemm_nt_acc<C, BK>(acc, A, ts, B, ts, (int)ts, lds, 1.0);
+  }
+  acc_update<C>(acc, tile_f64(mo, m, n) + bi * C::BM + (long)bj * C::BN * ts, ts, BK ? 1.0 : 0.0);
+}
+
+template <class C>
+static void fisher_cfg(const TileView& z, const TileView& d, double* Y, const TileView& mo, hipStream_t s) {
+  const long per = (long)(z.ts / C::BM) * (z.ts / C::BN);
+  const long full = (long)z.mt * z.mt * per;
+  const long lower = (long)z.mt * (z.mt + 1) / 2 * per;
+  hipLaunchKernelGGL((k_fisher_zd<C, false, false>), dim3((unsigned)full), dim3(C::NT), 0, s, z, d, Y);
+  hipLaunchKernelGGL((k_fisher_zd<C, false, true>), dim3((unsigned)full), dim3(C::NT), 0, s, z, d, Y);
+  hipLaunchKernelGGL((k_fisher_zd<C, true, true>), dim3((unsigned)full), dim3(C::NT), 0, s, z, d, Y);
+  hipLaunchKernelGGL((k_fisher_zd<C, true, false>), dim3((unsigned)full), dim3(C::NT), 0, s, z, d, Y);
+  hipLaunchKernelGGL((k_fisher_yz<C, false>), dim3((unsigned)lower), dim3(C::NT), 0, s, (const double*)Y, z, mo);
+  hipLaunchKernelGGL((k_fisher_yz<C, true>), dim3((unsigned)lower), dim3(C::NT), 0, s, (const double*)Y, z, mo);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_gen_dmatern(const TileView& d, const MaternGradParams& P, int p, hipStream_t s) {
+  hipLaunchKernelGGL(k_gen_dmatern, dim3((unsigned)grad_partial_blocks(d)), dim3(64, 4), 0, s, d, P, p);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_fisher_zdz(const TileView& z, const TileView& d, double* Y, const TileView& mo, hipStream_t s) {
+  if (z.ts % 192 == 0) fisher_cfg<C192>(z, d, Y, mo, s);
+  else if (z.ts % 128 == 0) fisher_cfg<C128>(z, d, Y, mo, s);
+  else fisher_cfg<CT64>(z, d, Y, mo, s);
+}
+
 // out[p * G + g] = sum over b = g, g + G, ... < nb of partial[b*stride + p].
 __global__ void __launch_bounds__(256) k_reduce_partials(const double* partial, long nb, int stride, double* out) {
   __shared__ double red[4];

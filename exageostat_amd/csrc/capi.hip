@@ -940,6 +940,78 @@ long exageostat_mixed_prec_map(MLE_data* data, unsigned char* out, long cap) {
   }, -1L);
 }
 
+int exageostat_fisher(MLE_data* data, const double* theta, double* fisher, double* fisher_inv) {
+  return guarded([&]() {
+    Context* ctx = ctx_of(data);
+    const KernelId kid = kernel_from_name(data->kernel_fun);
+    if (kid != kUnivariate && kid != kNugget)
+      throw std::invalid_argument("Fisher information: univariate Matern kernels only");
+    ctx->l1.upload(data->l1.x, data->l1.y, ctx->N, ctx->s1);
+    const int np = kernel_num_params(kid);
+    std::vector<double> F((size_t)np * np);
+    const int info = fisher_information(*ctx, kid, theta, metric_of(data->dm), F.data());
+    data->status = info;
+    if (info != 0) {
+      set_error(info, "covariance matrix is not positive definite");
+      return info;
+    }
+    if (fisher) std::copy(F.begin(), F.end(), fisher);
+    // F^-1 through the Cholesky factor of the small np x np matrix
+    std::vector<double> L((size_t)np * np, 0.0), inv((size_t)np * np, 0.0);
+    for (int j = 0; j < np; ++j) {
+      double d = F[j * np + j];
+      for (int k = 0; k < j; ++k) d -= L[j * np + k] * L[j * np + k];
+      if (!(d > 0.0)) {
+        if (fisher_inv) std::fill(fisher_inv, fisher_inv + np * np, std::nan(""));
+        set_error(-2, "Fisher information matrix is not positive definite");
+        return -2;
+      }
+      L[j * np + j] = std::sqrt(d);
+      for (int i = j + 1; i < np; ++i) {
+        double v = F[i * np + j];
+        for (int k = 0; k < j; ++k) v -= L[i * np + k] * L[j * np + k];
+        L[i * np + j] = v / L[j * np + j];
+      }
+    }
+    for (int c = 0; c < np; ++c) {
+      std::vector<double> y(np, 0.0);  // L y = e_c, then L^T x = y
+      for (int i = 0; i < np; ++i) {
+        double v = i == c ? 1.0 : 0.0;
+        for (int k = 0; k < i; ++k) v -= L[i * np + k] * y[k];
+        y[i] = v / L[i * np + i];
+      }
+      for (int i = np - 1; i >= 0; --i) {
+        double v = y[i];
+        for (int k = i + 1; k < np; ++k) v -= L[k * np + i] * inv[k * np + c];
+        inv[i * np + c] = v / L[i * np + i];
+      }
+    }
+    if (fisher_inv) std::copy(inv.begin(), inv.end(), fisher_inv);
+    return 0;
+  }, -1);
+}
+
+double* EXAGEOSTAT_Fisher_Tile(MLE_data* data, int N, double* initial_theta, int dts, int p_grid, int q_grid) {
+  (void)dts;
+  (void)p_grid;
+  (void)q_grid;
+  return guarded([&]() -> double* {
+    Context* ctx = ctx_of(data);
+    if (N != ctx->N) throw std::invalid_argument("Fisher: N differs from the context's size");
+    const KernelId kid = kernel_from_name(data->kernel_fun);
+    const int np = (kid == kUnivariate || kid == kNugget) ? kernel_num_params(kid) : 3;
+    std::vector<double> inv((size_t)np * np);
+    const double t0 = wall_seconds();
+    if (exageostat_fisher(data, initial_theta, nullptr, inv.data()) != 0) return nullptr;
+    const double t = wall_seconds() - t0;
+    double* r = static_cast<double*>(calloc((size_t)np * np + 1, sizeof(double)));
+    if (!r) throw std::runtime_error("Fisher: out of host memory");
+    std::copy(inv.begin(), inv.end(), r);
+    if (!data->silent) fprintf(stderr, " ---- Time to calculate Fisher matrix: %6.4f seconds\n", t);
+    return r;
+  }, static_cast<double*>(nullptr));
+}
+
 void exageostat_gpu_profile_enable(int on) { Profiler::get().enable(on != 0); }
 
 void exageostat_gpu_profile_reset(void) { Profiler::get().reset(); }

@@ -131,6 +131,12 @@ struct BivariateGradParams {
 // Number of partial blocks of launch_grad_matern / launch_grad_bivariate
 // (partial: 4, resp. 8, doubles per block).
 long grad_partial_blocks(const TileView& v);
+// dSigma/dtheta_p of the univariate Matern kernels into the lower tiles of d
+// (diagonal tiles full, zero outside the logical matrix).
+void launch_gen_dmatern(const TileView& d, const MaternGradParams& P, int p, hipStream_t s);
+// M = Z D Z (lower tiles of mo) for symmetric Z and D held in lower tiles,
+// through the full tile matrix Y = Z D (mt * mt tiles of scratch).
+void launch_fisher_zdz(const TileView& z, const TileView& d, double* Y, const TileView& mo, hipStream_t s);
 // out[p*groups + g], p < 4: partial sums of sum_ij w_ij (Z_ij - a_i a_j)
 // dSigma_ij/dtheta_p over the lower tiles of Z (w = 1 on the diagonal, 2
 // elsewhere); p = sigma^2, beta, nu, nugget.

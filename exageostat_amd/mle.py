@@ -208,6 +208,21 @@ class ExactMLE:
         return {"gen": d.time_gen, "facto": d.time_facto, "logdet": d.time_logdet, "solve": d.time_solve,
                 "total": d.time_total}
 
+    # -- Fisher information ----------------------------------------------
+    def fisher(self, theta):
+        """Fisher information F (np x np) and its inverse at theta
+        (exageostat_fisher; EXAGEOSTAT_Fisher_Tile returns the inverse)."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        npar = len(th)
+        F = np.zeros((npar, npar))
+        Finv = np.zeros((npar, npar))
+        st = self.lib.exageostat_fisher(ctypes.byref(self.data), _dp(th), _dp(F), _dp(Finv))
+        if st == -1:
+            _check(self.lib, "exageostat_fisher")
+        # st > 0: Sigma(theta) not positive definite (data.status); st == -2:
+        # F not positive definite, Finv is NaN
+        return F, Finv
+
     # -- prediction ------------------------------------------------------
     def predict(self, theta, xobs, yobs, zobs, xmiss, ymiss, zactual=None):
         """EXAGEOSTAT_dmle_Predict_Tile; returns (zmiss, mse)."""

@@ -205,6 +205,21 @@ double MORSE_dmle_Predict_Tile(MLE_data *data, double *theta, int nZmiss, int nZ
 void prediction_init(MLE_data *data, int nZmiss, int nZobs, int ts, int p_grid, int q_grid, int mse_flag);
 void prediction_finalize(MLE_data *data);
 
+/* ------------------------------------------------- Fisher information -- */
+/* exageostat_exact/src/compute/MLE_dexact.c:1777-1934 (declared in
+ * MLE_exact.h).  Fisher information F_jk = 1/2 tr(Sigma^-1 dSigma_j
+ * Sigma^-1 dSigma_k) at initial_theta over data->l1 (N locations; dts,
+ * p_grid, q_grid as in the reference, the context's tile size is used);
+ * returns a malloc'd array of np*np + 1 doubles holding F^-1 column-major
+ * (np = 3 for "univariate_matern_stationary" as in the reference, 4 with
+ * the nugget kernel), or NULL (status in exageostat_gpu_last_status). */
+double *EXAGEOSTAT_Fisher_Tile(MLE_data *data, int N, double *initial_theta, int dts, int p_grid, int q_grid);
+/* No counterpart in the reference.  F and F^-1 (np x np each, symmetric)
+ * at theta; returns 0, > 0 (Sigma(theta) not positive definite: first
+ * failing column + 1), -1 (error) or -2 (F not positive definite; F^-1 is
+ * then NaN). */
+int exageostat_fisher(MLE_data *data, const double *theta, double *fisher, double *fisher_inv);
+
 /* ------------------------------------------- descriptors and sequences -- */
 /* Stand-ins for the Chameleon calls the reference makes on this path
  * (CHAMELEON_Desc_Create via EXAGEOSTAT_ALLOCATE_MATRIX_TILE, MLE_misc.h:81-87;

@@ -97,6 +97,28 @@ def gradient(kernel: str, x, y, z, theta, metric: int = 0):
     return l, g
 
 
+def fisher(kernel: str, x, y, theta, metric: int = 0):
+    """Fisher information F_jk = 1/2 tr(S^-1 dS_j S^-1 dS_k) of the
+    univariate Matern kernels, as EXAGEOSTAT_Fisher_Tile forms it
+    (exageostat_exact/src/compute/MLE_dexact.c:1777-1934: two triangular
+    solves of each derivative matrix with the Cholesky factor, then the
+    trace of each product, A[k + 3 j] = 0.5 t for k >= j).  Returns (F, F^-1)
+    with F^-1 through the Cholesky factor of F (:1886-1921)."""
+    nugget = kernel == "univariate_matern_nuggets_stationary"
+    S = sigma(kernel, x, y, theta, metric)
+    cf = sla.cho_factor(S, lower=True)
+    dS = matern.matern_derivatives(x, y, x, y, theta, metric, nugget=nugget)
+    B = [sla.cho_solve(cf, D) for D in dS]           # S^-1 dS_j
+    npar = len(dS)
+    F = np.zeros((npar, npar))
+    for j in range(npar):
+        for k in range(j, npar):
+            F[j, k] = F[k, j] = 0.5 * float(np.sum(B[j] * B[k].T))   # tr(B_j B_k)
+    Lf = np.linalg.cholesky(F)
+    Finv = sla.cho_solve((Lf, True), np.eye(npar))
+    return F, Finv
+
+
 def flops_dpotrf(n: float) -> float:
     """FLOPS_DPOTRF (include/flops.h:97-98,254)."""
     n = float(n)

@@ -151,3 +151,66 @@ def test_lbfgsb_reaches_derivative_free_theta(lib, c1):
     assert abs(r_g.loglik - r_d.loglik) <= 1e-7 * abs(r_d.loglik), (r_g.loglik, r_d.loglik)
     assert np.max(np.abs(r_g.theta - r_d.theta) / np.abs(r_d.theta)) <= 1e-3, (r_g.theta, r_d.theta)
     assert r_g.evals * 3 < r_d.evals, (r_g.evals, r_d.evals)
+
+
+@pytest.mark.parametrize("theta", [(1.0, 0.1, 0.5), (1.3, 0.07, 0.8)])
+def test_fisher_matches_oracle(lib, c1, theta):
+    """C1 locations (n = 1,600, ts = 320): F and F^-1 against the oracle
+    (itself pinned by closed forms and the covariance of the score)."""
+    F_ref, Finv_ref = likelihood.fisher(K, c1["x"], c1["y"], theta)
+    m = lib.ExactMLE(c1["x"], c1["y"], 320)
+    m.set_z(c1["z"])
+    F, Finv = m.fisher(theta)
+    m.close()
+    assert np.max(np.abs(F - F_ref)) <= 1e-9 * np.max(np.abs(F_ref)), (F, F_ref)
+    assert np.max(np.abs(Finv - Finv_ref)) <= 1e-7 * np.max(np.abs(Finv_ref)), (Finv, Finv_ref)
+
+
+def test_fisher_nugget_ragged(lib):
+    """Nugget kernel (4 x 4), n = 1,100 with ts = 128 (ragged last tile)."""
+    x, y, _ = datagen.generate_xy_loc(1100, 4)
+    theta = (1.0, 0.1, 0.5, 0.05)
+    KN_ = "univariate_matern_nuggets_stationary"
+    F_ref, _ = likelihood.fisher(KN_, x, y, theta)
+    m = lib.ExactMLE(x, y, 128, kernel=KN_)
+    m.set_z(np.zeros(len(x)))
+    F, Finv = m.fisher(theta)
+    m.close()
+    assert F.shape == (4, 4)
+    assert np.max(np.abs(F - F_ref)) <= 1e-9 * np.max(np.abs(F_ref)), (F, F_ref)
+    np.testing.assert_allclose(F @ Finv, np.eye(4), atol=1e-8)
+
+
+def test_fisher_tile_entry_point(lib, c1):
+    """EXAGEOSTAT_Fisher_Tile (the reference's signature) returns F^-1 as a
+    malloc'd array of 3 * 3 + 1 doubles."""
+    import ctypes
+    theta = np.array([1.0, 0.1, 0.5])
+    _, Finv_ref = likelihood.fisher(K, c1["x"], c1["y"], theta)
+    m = lib.ExactMLE(c1["x"], c1["y"], 320)
+    m.set_z(c1["z"])
+    r = m.lib.EXAGEOSTAT_Fisher_Tile(ctypes.byref(m.data), 1600, theta.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                     320, 1, 1)
+    assert r
+    got = np.array([r[i] for i in range(10)])
+    ctypes.CDLL(None).free(ctypes.cast(r, ctypes.c_void_p))
+    m.close()
+    assert got[9] == 0.0
+    Finv = got[:9].reshape(3, 3, order="F")
+    assert np.max(np.abs(Finv - Finv_ref)) <= 1e-7 * np.max(np.abs(Finv_ref))
+
+
+def test_fisher_large_closed_forms(lib):
+    """n = 12,000 (ts = 960): F_00 = n / (2 sigma^4) exactly, and F_0k equals
+    -g_k / sigma^2 for the adjoint gradient g at z = 0 (two GPU paths)."""
+    n, ts = 12000, 960
+    theta = np.array([1.2, 0.1, 0.7])
+    x, y, _ = datagen.generate_xy_loc(n, 2)
+    m = lib.ExactMLE(x, y, ts)
+    m.set_z(np.zeros(n))
+    F, Finv = m.fisher(theta)
+    _, g = m.loglik(theta, grad=True)
+    m.close()
+    assert abs(F[0, 0] - n / (2 * theta[0] ** 2)) <= 1e-9 * F[0, 0], F[0, 0]
+    np.testing.assert_allclose(F[0], -g / theta[0], rtol=1e-8)
+    np.testing.assert_allclose(F @ Finv, np.eye(3), atol=1e-7)

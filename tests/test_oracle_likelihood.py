@@ -133,3 +133,40 @@ def test_bivariate_golden_and_spd():
 def test_flop_formulas():
     assert likelihood.flops_dpotrf(90000) == pytest.approx(2.430e14, rel=1e-3)
     assert likelihood.flops_dtrsm_left(90000, 1) == 90000 * 90001
+
+
+def test_fisher_closed_forms():
+    """Fisher information pinned without its own formula: with dS/dsigma^2 =
+    S/sigma^2 (no nugget), F_00 = n / (2 sigma^4) and F_0k = -g_k(z = 0) /
+    sigma^2, where g is the oracle gradient (pinned to finite differences) at
+    z = 0, whose value is -1/2 tr(S^-1 dS_k); with the nugget kernel
+    (dS/dtau^2 = I), F_33 = 1/2 sum 1/lambda_i^2 over the eigenvalues of S."""
+    x, y, _ = datagen.generate_xy_loc(300, 3)
+    theta = (1.3, 0.08, 0.9)
+    F, Finv = likelihood.fisher("univariate_matern_stationary", x, y, theta)
+    n = len(x)
+    assert abs(F[0, 0] - n / (2 * theta[0] ** 2)) <= 1e-10 * F[0, 0]
+    _, g0 = likelihood.gradient("univariate_matern_stationary", x, y, np.zeros(n), theta)
+    np.testing.assert_allclose(F[0], -g0 / theta[0], rtol=1e-9)
+    np.testing.assert_allclose(F @ Finv, np.eye(3), atol=1e-9)
+    thn = (1.0, 0.1, 0.5, 0.05)
+    Fn, _ = likelihood.fisher("univariate_matern_nuggets_stationary", x, y, thn)
+    lam = np.linalg.eigvalsh(likelihood.sigma("univariate_matern_nuggets_stationary", x, y, thn))
+    assert abs(Fn[3, 3] - 0.5 * np.sum(lam ** -2.0)) <= 1e-9 * Fn[3, 3]
+
+
+def test_fisher_is_covariance_of_the_score():
+    """F = E[s s^T] for the score s = d l / d theta at z ~ N(0, S): a Monte
+    Carlo estimate from the oracle gradient (2,000 draws, n = 36) agrees
+    with every entry within 4.5 standard errors."""
+    x, y, _ = datagen.generate_xy_loc(36, 7)
+    theta = (1.0, 0.2, 0.8)
+    K = "univariate_matern_stationary"
+    F, _ = likelihood.fisher(K, x, y, theta)
+    L = np.linalg.cholesky(likelihood.sigma(K, x, y, theta))
+    rng = np.random.default_rng(5)
+    S = np.array([likelihood.gradient(K, x, y, L @ rng.standard_normal(len(x)), theta)[1] for _ in range(2000)])
+    est = S.T @ S / len(S)
+    prod = np.einsum("si,sj->sij", S, S)
+    se = prod.std(axis=0) / np.sqrt(len(S))
+    assert np.all(np.abs(est - F) <= 4.5 * se), (est, F, se)
