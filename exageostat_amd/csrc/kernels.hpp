@@ -84,7 +84,8 @@ void launch_convert_mixed(const TileView& src, const TileView& dst, long ntiles,
 // classes: tile m at split + 2 m ts^2 (mt tiles of room).
 void launch_split_panel(const TileView& v, int k, uint16_t* split, hipStream_t s);
 // Trailing update of step k over the tile pairs (packed m << 16 | n) of the
-// three precision classes of the updated tile, in one launch: pairs[0] /
+// three precision classes of the updated tile, in one launch when ts is a
+// multiple of 192 (else one launch per class): pairs[0] /
 // nblocks[0] FP64 tiles (FP64 MFMA), [1] FP32 tiles (split-BF16 MFMA, with
 // the lo*lo product when lolo), [2] far tiles (BF16 MFMA).  nblocks counts
 // tile pairs; the BF16 classes read their operands from the split panel of

@@ -172,6 +172,65 @@ __device__ __forceinline__ void class_update(const TileView& v, int k, int m, in
   }
 }
 
+// One precision class per launch (tile sizes that are not a multiple of 192:
+// the 128 x 128 and 64 x 64 block shapes).  Work-group w runs on XCD w % nx
+// and the XCDs take contiguous slices of the class's block list.
+template <int CLS, int BS>
+__global__ void __launch_bounds__((ClassCfg<CLS, BS>::C::NT))
+    k_update_class(TileView v, int k, const int* pairs, long nblocks, int nx, const uint16_t* split) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[ClassCfg<CLS, BS>::LDS_BYTES];
+  const int q = v.ts / BS;
+  const int w = blockIdx.x;
+  const int x = w % nx;
+  const int gx = gridDim.x / nx;
+  const long per = (nblocks + nx - 1) / nx;
+  const long start = (long)x * per;
+  const long end = start + per < nblocks ? start + per : nblocks;
+  for (long b = start + w / nx; b < end; b += gx) {
+    const long p = b / (q * q);
+    const int rem = (int)(b - p * (q * q));
+    const int bi = rem % q;
+    const int bj = rem / q;
+    const int pk = pairs[p];
+    const int m = pk >> 16;
+    const int n = pk & 0xFFFF;
+    if (m == n && bi < bj) continue;  // strictly upper block of a diagonal tile
+    class_update<CLS, BS>(v, k, m, n, bi, bj, split, lds);
+  }
+}
+
+template <int CLS, int BS>
+static void launch_class_bs(const TileView& v, int k, const int* pairs, long npairs, const uint16_t* split,
+                            hipStream_t s) {
+  static int cap = 0;
+  if (!cap) {
+    int dev = 0, cus = 0, occ = 1;
+    EXA_HIP(hipGetDevice(&dev));
+    EXA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    EXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_update_class<CLS, BS>, ClassCfg<CLS, BS>::C::NT, 0));
+    int reserve = 16;
+    if (const char* e = getenv("EXAGEOSTAT_RESERVE_CUS")) reserve = atoi(e);
+    if (reserve < 0 || reserve >= cus) reserve = 0;
+    cap = (cus - reserve) * (occ > 0 ? occ : 1);
+    if (cap >= 8) cap -= cap % 8;
+    if (cap < 1) cap = 1;
+  }
+  const long q = v.ts / BS;
+  const long blocks = npairs * q * q;
+  const int grid = blocks < cap ? (int)blocks : cap;
+  const int nx = (grid % 8 == 0) ? 8 : 1;
+  hipLaunchKernelGGL((k_update_class<CLS, BS>), dim3((unsigned)grid), dim3(ClassCfg<CLS, BS>::C::NT), 0, s, v, k,
+                     pairs, blocks, nx, split);
+  EXA_HIP(hipGetLastError());
+}
+
+template <int BS, int F32CLS>
+static void launch_per_class(const TileView& v, int k, const FusedSpans& sp, const uint16_t* split, hipStream_t s) {
+  if (sp.nblocks[0] > 0) launch_class_bs<kF64, BS>(v, k, sp.pairs[0], sp.nblocks[0], split, s);
+  if (sp.nblocks[1] > 0) launch_class_bs<F32CLS, BS>(v, k, sp.pairs[1], sp.nblocks[1], split, s);
+  if (sp.nblocks[2] > 0) launch_class_bs<kF32B, BS>(v, k, sp.pairs[2], sp.nblocks[2], split, s);
+}
+
 // The trailing update of one step over all three precision classes in one
 // persistent launch.  XCD x takes slice x of the FP64 class's block list,
 // then slice x of the FP32 list, then of the far list (the slowest blocks
@@ -268,11 +327,14 @@ static void launch_fused_bs(const TileView& v, int k, const FusedSpans& sp0, con
   EXA_HIP(hipGetLastError());
 }
 
+// 192 x 192 blocks: one fused launch.  The 128 and 64 block shapes keep one
+// launch per class: fused, every class inherits the FP64 path's register
+// allocation (measured 10% slower at ts = 1,024).
 template <int F32CLS>
 static void launch_fused(const TileView& v, int k, const FusedSpans& sp, const uint16_t* split, hipStream_t s) {
   if (v.ts % 192 == 0) launch_fused_bs<192, F32CLS>(v, k, sp, split, s);
-  else if (v.ts % 128 == 0) launch_fused_bs<128, F32CLS>(v, k, sp, split, s);
-  else launch_fused_bs<64, F32CLS>(v, k, sp, split, s);
+  else if (v.ts % 128 == 0) launch_per_class<128, F32CLS>(v, k, sp, split, s);
+  else launch_per_class<64, F32CLS>(v, k, sp, split, s);
 }
 
 void launch_update_fused(const TileView& v, int k, const FusedSpans& sp, const uint16_t* split, bool lolo,
