@@ -259,9 +259,11 @@ void launch_trtri_tiles(const TileView& v, const double* dinv, double* xinv, hip
 }
 
 // ---------------------------------------------------------------------------
-// Trailing update A(m,n) -= A(m,k) A(n,k)^T over BM x BN blocks.
-// col_lo == col_hi: the single tile column col_lo (tiles m >= col_lo);
-// otherwise the whole trailing triangle from column col_lo.
+// Trailing update A(m,n) -= sum_{p < np} A(m,k+p) A(n,k+p)^T over BM x BN
+// blocks (np = 2: two factorisation steps at once, K = 2 ts per block and
+// one read and write of C).
+// Tile pairs (m, n) with n in [col_lo, col_hi] and m in [n, mt), column by
+// column (consecutive pairs share the panel tile n).
 //
 // Persistent launch: gridDim.x work-groups (at most one per CU, fewer than
 // the CU count so the panel kernels of the critical path always find free
@@ -270,7 +272,7 @@ void launch_trtri_tiles(const TileView& v, const double* dinv, double* xinv, hip
 // contiguous slice of the block list and its work-groups walk it together,
 // which keeps the A/B row panels they share hot in that XCD's L2.
 template <class C>
-__device__ __forceinline__ void update_block(const TileView& v, int k, int col_lo, int col_hi, long b,
+__device__ __forceinline__ void update_block(const TileView& v, int k, int np, int col_lo, int col_hi, long b,
                                              double* lds) {
   const int qm = v.ts / C::BM;
   const int qn = v.ts / C::BN;
@@ -278,29 +280,35 @@ __device__ __forceinline__ void update_block(const TileView& v, int k, int col_l
   const int rem = (int)(b - p * (qm * qn));
   const int bi = rem % qm;
   const int bj = rem / qm;
-  int m, n;
-  if (col_lo == col_hi) {
-    n = col_lo;
-    m = col_lo + (int)p;
-  } else {
-    int r, c;
-    tri_decode(p, r, c);
-    m = col_lo + r;
-    n = col_lo + c;
+  // column n: the last c in [col_lo, col_hi] with pairs_before(c) <= p
+  const int mt = v.mt;
+  auto pairs_before = [&](int c) {
+    return (long)(c - col_lo) * mt - ((long)c * (c - 1) / 2 - (long)col_lo * (col_lo - 1) / 2);
+  };
+  int lo = col_lo, hi = col_hi;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (pairs_before(mid) <= p) lo = mid;
+    else hi = mid - 1;
   }
+  const int n = lo;
+  const int m = n + (int)(p - pairs_before(n));
   if (m == n && (bi + 1) * C::BM <= bj * C::BN) return;
-  const double* A = tile_f64(v, m, k) + bi * C::BM;
-  const double* B = tile_f64(v, n, k) + bj * C::BN;
   double* Cp = tile_f64(v, m, n) + bi * C::BM + (long)bj * C::BN * v.ts;
   dx4 acc[C::FM][C::FN];
   zero_acc<C>(acc);
-  gemm_nt_acc<C>(acc, A, v.ts, B, v.ts, v.ts, lds, -1.0);
+#pragma unroll 1
+  for (int p = 0; p < np; ++p) {
+    const double* A = tile_f64(v, m, k + p) + bi * C::BM;
+    const double* B = tile_f64(v, n, k + p) + bj * C::BN;
+    gemm_nt_acc<C>(acc, A, v.ts, B, v.ts, v.ts, lds, -1.0);
+  }
   acc_update<C>(acc, Cp, v.ts, 1.0);
 }
 
 template <class C>
-__global__ void __launch_bounds__(C::NT) k_tile_update(TileView v, int k, int col_lo, int col_hi, long nblocks,
-                                                      int nx) {
+__global__ void __launch_bounds__(C::NT) k_tile_update(TileView v, int k, int np, int col_lo, int col_hi,
+                                                      long nblocks, int nx) {
   __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
   const int w = blockIdx.x;
   const int x = w % nx;
@@ -308,7 +316,7 @@ __global__ void __launch_bounds__(C::NT) k_tile_update(TileView v, int k, int co
   const long per = (nblocks + nx - 1) / nx;
   const long start = (long)x * per;
   const long end = start + per < nblocks ? start + per : nblocks;
-  for (long b = start + w / nx; b < end; b += gx) update_block<C>(v, k, col_lo, col_hi, b, lds);
+  for (long b = start + w / nx; b < end; b += gx) update_block<C>(v, k, np, col_lo, col_hi, b, lds);
 }
 
 static int g_update_wgs = 0;
@@ -334,14 +342,9 @@ static int update_grid_cap() {
 }
 
 template <class C>
-static void launch_update_cfg(const TileView& v, int k, int col_lo, int col_hi, hipStream_t s) {
-  long pairs;
-  if (col_lo == col_hi) {
-    pairs = v.mt - col_lo;
-  } else {
-    const long M = v.mt - col_lo;
-    pairs = M * (M + 1) / 2;
-  }
+static void launch_update_cfg(const TileView& v, int k, int np, int col_lo, int col_hi, hipStream_t s) {
+  long pairs = 0;
+  for (int c = col_lo; c <= col_hi && c < v.mt; ++c) pairs += v.mt - c;
   if (pairs <= 0) return;
   const long blocks = pairs * (v.ts / C::BM) * (v.ts / C::BN);
   const int cap = update_grid_cap<C>();
@@ -349,18 +352,21 @@ static void launch_update_cfg(const TileView& v, int k, int col_lo, int col_hi, 
   int nx = 8;
   if (grid % 8 != 0) nx = 1;
   g_update_wgs = grid;
-  hipLaunchKernelGGL(k_tile_update<C>, dim3((unsigned)grid), dim3(C::NT), 0, s, v, k, col_lo, col_hi, blocks, nx);
+  hipLaunchKernelGGL(k_tile_update<C>, dim3((unsigned)grid), dim3(C::NT), 0, s, v, k, np, col_lo, col_hi, blocks,
+                     nx);
   EXA_HIP(hipGetLastError());
 }
 
-void launch_tile_update(const TileView& v, int k, int col_lo, int col_hi, hipStream_t s) {
+void launch_tile_update(const TileView& v, int k, int np, int col_lo, int col_hi, hipStream_t s) {
+  if (col_hi >= v.mt) col_hi = v.mt - 1;
   if (col_lo >= v.mt || col_hi < col_lo) return;
+  if (np < 1 || np > 2 || col_lo < k + np) throw std::invalid_argument("tile_update: bad panel count or columns");
   if (v.ts % 192 == 0) {
-    launch_update_cfg<GemmCfg<192, 192, 2, 4, 16>>(v, k, col_lo, col_hi, s);
+    launch_update_cfg<GemmCfg<192, 192, 2, 4, 16>>(v, k, np, col_lo, col_hi, s);
   } else if (v.ts % 128 == 0) {
-    launch_update_cfg<GemmCfg<128, 128, 2, 2, 16>>(v, k, col_lo, col_hi, s);
+    launch_update_cfg<GemmCfg<128, 128, 2, 2, 16>>(v, k, np, col_lo, col_hi, s);
   } else {
-    launch_update_cfg<C64>(v, k, col_lo, col_hi, s);
+    launch_update_cfg<C64>(v, k, np, col_lo, col_hi, s);
   }
 }
 

@@ -392,7 +392,7 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
   };
   auto update = [&](int k, int lo, int hi, hipStream_t s) {
     if (!lists) {
-      profiled("tile_update", update_flops(A, k, lo, hi), 0.0, s, [&] { launch_tile_update(v, k, lo, hi, s); });
+      profiled("tile_update", update_flops(A, k, lo, hi), 0.0, s, [&] { launch_tile_update(v, k, 1, lo, hi, s); });
       return;
     }
     // One fused launch for the three classes; each class's share of the
@@ -415,31 +415,95 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
     profiled("tile_update_mixed", fl, 0.0, s,
              [&] { launch_update_fused(v, k, sp, lists->split[k & 1], lists->lolo, s); });
   };
+  int nev = 0;
+  auto record = [&](hipStream_t st) {
+    hipEvent_t e = ws.ev(nev++);
+    EXA_HIP(hipEventRecord(e, st));
+    return e;
+  };
   // s1 carries the panel factorisations (small kernels on the CUs the
   // persistent update leaves free); s2 carries the trailing updates: first
-  // column k+1 of step k (which unblocks panel k+1), then the rest of step k,
-  // which overlaps with panel k+1 on s1.
-  EXA_HIP(hipEventRecord(ws.ev(0), s1));
-  EXA_HIP(hipStreamWaitEvent(s2, ws.ev(0), 0));
+  // the column that unblocks the next panel, then the rest, which overlaps
+  // with that panel on s1.
+  EXA_HIP(hipStreamWaitEvent(s2, record(s1), 0));
   factor_panel(0);
-  EXA_HIP(hipEventRecord(ws.ev(1), s1));
-  for (int k = 0; k + 1 < mt; ++k) {
-    hipEvent_t ev_panel_k = ws.ev(1 + 2 * k);
-    hipEvent_t ev_panel_k1 = ws.ev(1 + 2 * (k + 1));
-    hipEvent_t ev_col_k = ws.ev(2 + 2 * k);
-    EXA_HIP(hipStreamWaitEvent(s2, ev_panel_k, 0));
-    update(k, k + 1, k + 1, s2);
-    EXA_HIP(hipEventRecord(ev_col_k, s2));
-    if (k + 2 < mt) update(k, k + 2, mt - 1, s2);
-    EXA_HIP(hipStreamWaitEvent(s1, ev_col_k, 0));
-    factor_panel(k + 1);
-    EXA_HIP(hipEventRecord(ev_panel_k1, s1));
+  hipEvent_t ev_panel = record(s1);
+  if (lists) {
+    for (int k = 0; k + 1 < mt; ++k) {
+      EXA_HIP(hipStreamWaitEvent(s2, ev_panel, 0));
+      update(k, k + 1, k + 1, s2);
+      hipEvent_t ev_col = record(s2);
+      if (k + 2 < mt) update(k, k + 2, mt - 1, s2);
+      EXA_HIP(hipStreamWaitEvent(s1, ev_col, 0));
+      factor_panel(k + 1);
+      ev_panel = record(s1);
+    }
+  } else {
+    // FP64 factor: the trailing matrix is updated with two panels at once
+    // (K = 2 ts per block, one read and write of C per two steps).  For the
+    // pair (k, k+1):
+    //   s2: column k+1 with panel k;            s1: panel k+1
+    //   s2: the deferred part of the previous pair's trailing update
+    //   s2: column k+2 with panels k, k+1;      s1: panel k+2
+    //   s2: column k+3, then the first part of columns >= k+4 with panels
+    //       k, k+1; the rest is deferred to run during panel k+3.
+    // Updates of a column commute; each column receives all of its updates
+    // on s2 before its panel starts on s1, and every panel overlaps update
+    // work on s2.
+    auto update2 = [&](int kk, int lo, int hi) {
+      profiled("tile_update", update_flops(A, kk, lo, hi) + update_flops(A, kk + 1, lo, hi), 0.0, s2,
+               [&] { launch_tile_update(v, kk, 2, lo, hi, s2); });
+    };
+    struct Deferred {
+      int k = -1, lo = 0, hi = -1;
+    } pend;
+    auto run_pending = [&] {
+      if (pend.k >= 0 && pend.lo <= pend.hi) update2(pend.k, pend.lo, pend.hi);
+      pend.k = -1;
+    };
+    // columns [lo, hi] split into two parts of about equal update work
+    // (column c of the trailing matrix has mt - c tiles)
+    auto split_cols = [&](int lo, int hi) {
+      double total = 0.0;
+      for (int c = lo; c <= hi; ++c) total += mt - c;
+      double acc = 0.0;
+      int mid = lo;
+      for (; mid < hi; ++mid) {
+        acc += mt - mid;
+        if (acc >= 0.5 * total) break;
+      }
+      return mid;
+    };
+    int k = 0;
+    while (k + 1 < mt) {
+      EXA_HIP(hipStreamWaitEvent(s2, ev_panel, 0));
+      update(k, k + 1, k + 1, s2);
+      EXA_HIP(hipStreamWaitEvent(s1, record(s2), 0));
+      factor_panel(k + 1);
+      ev_panel = record(s1);
+      run_pending();
+      if (k + 2 >= mt) break;
+      EXA_HIP(hipStreamWaitEvent(s2, ev_panel, 0));
+      update2(k, k + 2, k + 2);
+      hipEvent_t ev_col = record(s2);
+      EXA_HIP(hipStreamWaitEvent(s1, ev_col, 0));
+      factor_panel(k + 2);
+      ev_panel = record(s1);
+      if (k + 3 < mt) update2(k, k + 3, k + 3);
+      if (k + 4 < mt) {
+        const int mid = split_cols(k + 4, mt - 1);
+        update2(k, k + 4, mid);
+        pend.k = k;
+        pend.lo = mid + 1;
+        pend.hi = mt - 1;
+      }
+      k += 2;
+    }
+    run_pending();
   }
   // Join: s1 waits for the last updates on s2, s2 for the last panel on s1.
-  EXA_HIP(hipEventRecord(ws.ev(2 * mt + 2), s2));
-  EXA_HIP(hipStreamWaitEvent(s1, ws.ev(2 * mt + 2), 0));
-  EXA_HIP(hipEventRecord(ws.ev(2 * mt + 3), s1));
-  EXA_HIP(hipStreamWaitEvent(s2, ws.ev(2 * mt + 3), 0));
+  EXA_HIP(hipStreamWaitEvent(s1, record(s2), 0));
+  EXA_HIP(hipStreamWaitEvent(s2, record(s1), 0));
   launch_zero_upper_diag(v, s1);
   profiled("trtri_diag", (double)mt * ts * (double)ts * ts / 3.0, 0.0, s1,
            [&] { launch_trtri_tiles(v, ws.dinv_base(), ws.xinv_base(), s1); });

@@ -46,8 +46,9 @@ void launch_gen_bivariate(const TileView& v, const BivariateParams& P, int first
 void launch_potrf_tile(double* T, int ts, double* dinv_k, int* info, long col_base, hipStream_t s);
 // A(m,k) <- A(m,k) L_kk^-T for all m in (k, mt).
 void launch_trsm_panel(const TileView& v, int k, const double* dinv_k, hipStream_t s);
-// A(m,n) -= A(m,k) A(n,k)^T for n in [col_lo, col_hi], m in [n, mt).
-void launch_tile_update(const TileView& v, int k, int col_lo, int col_hi, hipStream_t s);
+// A(m,n) -= sum_{p < np} A(m,k+p) A(n,k+p)^T for n in [col_lo, col_hi],
+// m in [n, mt); np = 1 or 2 (col_lo >= k + np).
+void launch_tile_update(const TileView& v, int k, int np, int col_lo, int col_hi, hipStream_t s);
 
 // X_k = L_kk^-1 for every diagonal tile (xinv: mt tiles of ts x ts).
 void launch_trtri_tiles(const TileView& v, const double* dinv, double* xinv, hipStream_t s);
