@@ -329,7 +329,8 @@ def main() -> None:
         "traffic_unit": "GB/launch", "launches": upd["launches"], "avg_launch_ms": l64,
     }
     fp64_rec = {
-        "value": f64_value, "ms_per_step": 1000.0 * f64["elapsed"] / args.steps, "loglik": f64["ll"],
+        "value": f64_value, "ms_per_step": 1000.0 * f64["elapsed"] / args.steps,
+        "ms_per_step_excl_gen": 1000.0 * (f64["elapsed"] / args.steps - f64["phases"]["gen"]), "loglik": f64["ll"],
         "status": f64["status"], "phases_s": f64["phases"],
         "tflops_effective": eval_flops * f64_value / 1e12,
         "solve_check_rel": abs(f64["dotp"] - ee) / ee,
@@ -372,6 +373,9 @@ def main() -> None:
             "rel_dlogdet_vs_fp64": abs(mix["logdet"] - f64["logdet"]) / abs(f64["logdet"]),
             "rel_ddotp_vs_fp64": abs(mix["dotp"] - f64["dotp"]) / abs(f64["dotp"]),
             "speedup_vs_fp64": value / f64_value,
+            # the evaluation without Sigma's generation (and its conversion
+            # into the mixed factor)
+            "ms_per_step_excl_gen": 1000.0 * (mix["elapsed"] / args.steps - mix["phases"]["gen"]),
             "loglik": mix["ll"], "status": mix["status"], "phases_s": mix["phases"], "tiles": mix["tiles"],
             "mixed_options": MIXED, "update_classes": classes,
             "symv_GBps": (_rate(mix["prof"]["symv"], "bytes")[0] or 0) / 1e9,
