@@ -7,10 +7,13 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def build(jobs: int = 8, clean: bool = False) -> str:
+def build(jobs: int = 8, clean: bool = False, force: bool = False) -> str:
+    """Runs the Makefile; force recompiles every source even where the
+    objects are newer than their sources."""
     if clean:
         subprocess.run(["make", "-C", HERE, "clean"], check=True)
-    subprocess.run(["make", "-C", HERE, f"-j{jobs}"], check=True)
+    cmd = ["make", "-C", HERE, f"-j{jobs}"] + (["-B"] if force else [])
+    subprocess.run(cmd, check=True)
     return os.path.join(HERE, "libexageostat_gpu.so")
 
 
