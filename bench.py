@@ -249,6 +249,41 @@ def run_c5(steps: int) -> None:
     print(json.dumps(out), flush=True)
 
 
+def run_mloe(steps: int) -> None:
+    """MLOE / MMOM at n = 90,000: 1,000 held-out locations drawn at random
+    (seeded) from GenerateXYLoc(90000, 0), 89,000 observed, ts = 960,
+    theta_true = (1, 0.1, 0.5), theta_est = (0.9, 0.12, 0.55); one warm-up,
+    `steps` timed calls, then one call with the per-group kernel timing."""
+    import exageostat_amd
+    from exageostat_amd import mle
+    n, nmiss, ts = 90000, 1000, 960
+    x, y = exageostat_amd.generate_xy_loc(n, 0)
+    rng = np.random.default_rng(0)
+    miss = np.sort(rng.choice(n, nmiss, replace=False))
+    obs = np.setdiff1d(np.arange(n), miss)
+    xo, yo, xm, ym = x[obs], y[obs], x[miss], y[miss]
+    tt, te = (1.0, 0.1, 0.5), (0.9, 0.12, 0.55)
+    m = exageostat_amd.ExactMLE(xo, yo, ts)
+    m.mloe_mmom(tt, te, xo, yo, xm, ym)
+    mle.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        mloe, mmom, loe, _ = m.mloe_mmom(tt, te, xo, yo, xm, ym)
+    dt = (time.perf_counter() - t0) / steps
+    mle.profile_enable(True)
+    m.mloe_mmom(tt, te, xo, yo, xm, ym)
+    groups = {}
+    for g in ("tile_update", "mloe_solve", "mloe_symm"):
+        p = mle.profile_get(g)
+        groups[g] = {"launches": p["launches"], "ms": p["ms"],
+                     "tflops": p["flops"] / (p["ms"] * 1e-3) / 1e12 if p["ms"] > 0 else None}
+    mle.profile_enable(False)
+    m.close()
+    print(json.dumps({"config": "MLOE/MMOM: 89,000 observed, 1,000 held out, ts=960", "s_per_call": dt,
+                      "mloe": mloe, "mmom": mmom, "min_loe": float(np.min(loe)), "theta_true": tt,
+                      "theta_est": te, "groups": groups}), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,8 +294,9 @@ def main() -> None:
     ap.add_argument("--cpu-sample", type=int, default=16000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="time the FP64 path only")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c5"],
-                    help="c2 (default): the benchmark line; c3, c5: the other configurations' measurements")
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c5", "mloe"],
+                    help="c2 (default): the benchmark line; c3, c5: the other configurations' measurements; "
+                         "mloe: MLOE/MMOM at n = 90,000")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy kernel of BYTES (scales the HBM counters) and exit")
     args = ap.parse_args()
@@ -284,6 +320,9 @@ def main() -> None:
         return
     if args.config == "c5":
         run_c5(args.steps)
+        return
+    if args.config == "mloe":
+        run_mloe(args.steps)
         return
 
     import exageostat_amd

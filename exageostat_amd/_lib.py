@@ -54,7 +54,8 @@ class MLEData(ctypes.Structure):
         ("mserror1", ctypes.c_double), ("mserror2", ctypes.c_double), ("dm", ctypes.c_char_p),
         ("diag_thick", ctypes.c_int), ("opt_tol", ctypes.c_int), ("opt_max_iters", ctypes.c_int),
         ("kernel_fun", ctypes.c_char_p), ("precision", ctypes.c_int), ("mspe", ctypes.c_int),
-        ("seed", ctypes.c_int), ("gpu", ctypes.c_void_p), ("silent", ctypes.c_int),
+        ("seed", ctypes.c_int), ("mloe", ctypes.c_double), ("mmom", ctypes.c_double),
+        ("gpu", ctypes.c_void_p), ("silent", ctypes.c_int),
         ("status", ctypes.c_int), ("logdet", ctypes.c_double), ("time_gen", ctypes.c_double),
         ("time_facto", ctypes.c_double), ("time_logdet", ctypes.c_double), ("time_solve", ctypes.c_double),
         ("time_total", ctypes.c_double), ("mp_fp32_corr", ctypes.c_double), ("mp_bf16", ctypes.c_int),
@@ -112,6 +113,16 @@ _SIGNATURES = {
                                          _P(ctypes.c_double)]),
     "EXAGEOSTAT_Fisher_Tile": (_P(ctypes.c_double), [_P(MLEData), ctypes.c_int, _P(ctypes.c_double), ctypes.c_int,
                                                      ctypes.c_int, ctypes.c_int]),
+    "EXAGEOSTAT_dmle_mloe_mmom_Allocate": (None, [_P(MLEData)] + [ctypes.c_int] * 5),
+    "mloe_mmom_init": (None, [_P(MLEData)] + [ctypes.c_int] * 5),
+    "EXAGEOSTAT_dmle_mloe_mmom_Tile": (None, [_P(MLEData), c_double_p, c_double_p, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int]),
+    "EXAGEOSTAT_dmle_mloe_mmom_Tile_Async": (None, [_P(MLEData), c_double_p, c_double_p, ctypes.c_int,
+                                                    ctypes.c_int, ctypes.c_int]),
+    "mloe_mmom_finalize": (None, [_P(MLEData)]),
+    "MLOE_MMOM_Finalize": (None, [_P(MLEData)]),
+    "exageostat_mloe_mmom": (ctypes.c_int, [_P(MLEData), c_double_p, c_double_p, ctypes.c_int, ctypes.c_int,
+                                            c_double_p, c_double_p]),
     "exageostat_gpu_profile_get": (ctypes.c_int, [ctypes.c_char_p, _P(ctypes.c_long), c_double_p, c_double_p,
                                                   c_double_p]),
     "exageostat_gpu_last_status": (ctypes.c_int, []),

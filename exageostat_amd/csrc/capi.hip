@@ -18,6 +18,7 @@
 #include "engine.hpp"
 #include "mixed.hpp"
 #include "adjoint.hpp"
+#include "mloe.hpp"
 #include "profile.hpp"
 
 using namespace exa;
@@ -1011,6 +1012,84 @@ double* EXAGEOSTAT_Fisher_Tile(MLE_data* data, int N, double* initial_theta, int
     return r;
   }, static_cast<double*>(nullptr));
 }
+
+// ------------------------------------------------------------ MLOE / MMOM --
+void EXAGEOSTAT_dmle_mloe_mmom_Allocate(MLE_data* data, int nZmiss, int nZobs, int dts, int p_grid, int q_grid) {
+  (void)dts;
+  (void)p_grid;
+  (void)q_grid;
+  guarded([&]() {
+    if (nZmiss <= 0) throw std::invalid_argument("Number of missing values should be positive value");
+    if (nZobs <= 0) throw std::invalid_argument("Number of observed values should be positive value");
+    ctx_of(data);
+    data->mloe = 0.0;
+    data->mmom = 0.0;
+    return 0;
+  }, 0);
+}
+
+void mloe_mmom_init(MLE_data* data, int nZmiss, int nZobs, int ts, int p_grid, int q_grid) {
+  EXAGEOSTAT_dmle_mloe_mmom_Allocate(data, nZmiss, nZobs, ts, p_grid, q_grid);
+}
+
+int exageostat_mloe_mmom(MLE_data* data, const double* truth_theta, const double* estimated_theta, int nZmiss,
+                         int nZobs, double* loe, double* mom) {
+  return guarded([&]() {
+    Context* ctx = ctx_of(data);
+    const KernelId kid = kernel_from_name(data->kernel_fun);
+    if (kid == kUnknownKernel) throw std::invalid_argument("unsupported kernel");
+    if (kid == kBivariate) throw std::invalid_argument("MLOE/MMOM: univariate Matern kernels only");
+    if (nZmiss <= 0) throw std::invalid_argument("Number of missing values should be positive value");
+    if (nZobs <= 0) throw std::invalid_argument("Number of observed values should be positive value");
+    if (!data->lobs.x || !data->lobs.y || !data->lmiss.x || !data->lmiss.y)
+      throw std::invalid_argument("MLOE/MMOM: data->lobs and data->lmiss must hold the locations");
+    ctx->lobs.upload(data->lobs.x, data->lobs.y, nZobs, ctx->s1);
+    ctx->lmiss.upload(data->lmiss.x, data->lmiss.y, nZmiss, ctx->s1);
+    double mloe = 0.0, mmom = 0.0;
+    const int info = mloe_mmom(*ctx, kid, truth_theta, estimated_theta, metric_of(data->dm), nZobs, nZmiss, loe,
+                               mom, &mloe, &mmom);
+    data->status = info;
+    data->time_total = ctx->last.total;
+    if (info != 0) {
+      set_error(info, "MLOE/MMOM: covariance matrix is not positive definite");
+      data->mloe = data->mmom = std::nan("");
+      return info;
+    }
+    data->mloe = mloe;
+    data->mmom = mmom;
+    return 0;
+  }, -1);
+}
+
+void EXAGEOSTAT_dmle_mloe_mmom_Tile(MLE_data* data, double* truth_theta, double* estimated_theta, int nZmiss,
+                                    int nZobs, int n) {
+  (void)n;
+  const int rc = exageostat_mloe_mmom(data, truth_theta, estimated_theta, nZmiss, nZobs, nullptr, nullptr);
+  if (rc < 0 && data) data->status = rc;
+  if (rc == 0 && data && !data->silent) {
+    printf("\nnZmiss = %d\n", nZmiss);
+    printf("\nMMLOE = %f", data->mloe);
+    printf("\nMMOM = %f\n\n\n", data->mmom);
+    fprintf(stderr, " ---- mloe_mmom Time: %6.2f seconds\n\n", data->time_total);
+  }
+}
+
+void EXAGEOSTAT_dmle_mloe_mmom_Tile_Async(MLE_data* data, double* truth_theta, double* estimated_theta, int nZmiss,
+                                          int nZobs, int n) {
+  EXAGEOSTAT_dmle_mloe_mmom_Tile(data, truth_theta, estimated_theta, nZmiss, nZobs, n);
+}
+
+void mloe_mmom_finalize(MLE_data* data) {
+  guarded([&]() {
+    if (!data || !data->gpu) return 0;
+    Context* ctx = ctx_of(data);
+    destroy_mloe(ctx->mloe);
+    ctx->mloe = nullptr;
+    return 0;
+  }, 0);
+}
+
+void MLOE_MMOM_Finalize(MLE_data* data) { mloe_mmom_finalize(data); }
 
 void exageostat_gpu_profile_enable(int on) { Profiler::get().enable(on != 0); }
 

@@ -574,6 +574,8 @@ Context::Context(int N_, int ts_, int nZobs, int nZmiss) : N(N_), ts(ts_) {
 Context::~Context() {
   destroy_mixed(mixed);
   destroy_adjoint(adjoint);
+  destroy_mloe(mloe);
+  mloe = nullptr;
   mixed = nullptr;
   if (d_partial) (void)hipFree(d_partial);
   if (s1) (void)hipStreamDestroy(s1);

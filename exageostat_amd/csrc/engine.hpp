@@ -159,6 +159,8 @@ struct MixedState;
 void destroy_mixed(MixedState* m);
 struct AdjointState;
 void destroy_adjoint(AdjointState* a);
+struct MloeState;
+void destroy_mloe(MloeState* m);
 
 // Everything one MLE_data needs on the device.
 class Context {
@@ -180,6 +182,7 @@ class Context {
   PhaseTimes last;
   MixedState* mixed = nullptr;  // mixed-precision factor, created on first use
   AdjointState* adjoint = nullptr;  // gradient buffers, created on first use
+  MloeState* mloe = nullptr;        // MLOE / MMOM buffers, created on first use
 
   double loglik(KernelId kid, const double* theta, int metric, double* logdet_out, double* dotp_out, int* info);
   void ensure_partial(long n);

@@ -139,6 +139,18 @@ void launch_gen_dmatern(const TileView& d, const MaternGradParams& P, int p, hip
 // M = Z D Z (lower tiles of mo) for symmetric Z and D held in lower tiles,
 // through the full tile matrix Y = Z D (mt * mt tiles of scratch).
 void launch_fisher_zdz(const TileView& z, const TileView& d, double* Y, const TileView& mo, hipStream_t s);
+
+// MLOE / MMOM (mloe_kernels.hip).  r and o: rectangular tile matrices whose
+// rows are right-hand sides (r.nt == l.mt); l: lower tile factor, xinv: its
+// inverted diagonal tiles.  fwd: o = rows of r mapped by L^-1 (r <- r L^-T),
+// else by L^-T; r is overwritten.
+void launch_rhs_solve(const TileView& l, const double* xinv, const TileView& r, const TileView& o, bool fwd,
+                      hipStream_t s);
+// y = w S for the symmetric S in the lower tiles of sm (diagonal tiles full).
+void launch_rect_symm(const TileView& w, const TileView& sm, const TileView& y, hipStream_t s);
+// out[row] = sum over the logical columns of a(row, col) b(row, col), for
+// all a.mt * ts rows; part: a.nt * a.mt * ts doubles.
+void launch_row_dot(const TileView& a, const TileView& b, double* part, double* out, hipStream_t s);
 // out[p*groups + g], p < 4: partial sums of sum_ij w_ij (Z_ij - a_i a_j)
 // dSigma_ij/dtheta_p over the lower tiles of Z (w = 1 on the diagonal, 2
 // elsewhere); p = sigma^2, beta, nu, nugget.

@@ -248,9 +248,32 @@ class ExactMLE:
         _check(self.lib, "Predict_Tile")
         return zm, mse
 
+    def mloe_mmom(self, theta_true, theta_est, xobs, yobs, xmiss, ymiss):
+        """EXAGEOSTAT_dmle_mloe_mmom_Tile through exageostat_mloe_mmom;
+        returns (mloe, mmom, loe, mom) with the per-location values."""
+        xo = np.ascontiguousarray(xobs, dtype=np.float64)
+        yo = np.ascontiguousarray(yobs, dtype=np.float64)
+        xm = np.ascontiguousarray(xmiss, dtype=np.float64)
+        ym = np.ascontiguousarray(ymiss, dtype=np.float64)
+        self._mloe_refs = [xo, yo, xm, ym]
+        d = self.data
+        d.lobs.x, d.lobs.y, d.lobs.z = _dp(xo), _dp(yo), None
+        d.lmiss.x, d.lmiss.y, d.lmiss.z = _dp(xm), _dp(ym), None
+        tt = np.ascontiguousarray(theta_true, dtype=np.float64)
+        te = np.ascontiguousarray(theta_est, dtype=np.float64)
+        loe = np.zeros(len(xm))
+        mom = np.zeros(len(xm))
+        self.lib.mloe_mmom_init(ctypes.byref(d), len(xm), len(xo), self.ts, 1, 1)
+        _check(self.lib, "mloe_mmom_init")
+        rc = self.lib.exageostat_mloe_mmom(ctypes.byref(d), _dp(tt), _dp(te), len(xm), len(xo), _dp(loe), _dp(mom))
+        if rc < 0:
+            _check(self.lib, "exageostat_mloe_mmom")
+        return d.mloe, d.mmom, loe, mom
+
     def close(self) -> None:
         if self.data is not None and self.data.gpu:
             self.lib.prediction_finalize(ctypes.byref(self.data))
+            self.lib.mloe_mmom_finalize(ctypes.byref(self.data))
             self.lib.MLE_Finalize(ctypes.byref(self.data))
         self.data = None
 

@@ -121,6 +121,8 @@ typedef struct {
   int precision;        /* 0 double, 2 mixed (FP64 / FP32 / BF16 tiles)      */
   int mspe;
   int seed;
+  double mloe;          /* MLOE of the last EXAGEOSTAT_dmle_mloe_mmom_Tile     */
+  double mmom;          /* MMOM of the last EXAGEOSTAT_dmle_mloe_mmom_Tile     */
   /* ---- fields of this implementation ---- */
   void *gpu;            /* engine context (EXAGEOSTAT_dmle_Call / MLE_Finalize) */
   int silent;           /* 1: no per-iteration line on stderr                */
@@ -204,6 +206,31 @@ double MORSE_dmle_Predict_Tile(MLE_data *data, double *theta, int nZmiss, int nZ
 /* src/compute/MLE.c:284-306, 341-380. */
 void prediction_init(MLE_data *data, int nZmiss, int nZobs, int ts, int p_grid, int q_grid, int mse_flag);
 void prediction_finalize(MLE_data *data);
+
+/* ---------------------------------------------------------- MLOE / MMOM -- */
+/* MLE_exact.h:69-76, MLE_dexact.c:1002-1435; src/compute/MLE.c:309-320,403.
+ * Prediction efficiency of kriging with estimated_theta when truth_theta
+ * holds, over the held-out locations data->lmiss (nZmiss) given
+ * data->lobs (nZobs); univariate Matern kernels.  Results in data->mloe and
+ * data->mmom (the means over the held-out locations of E2/E3 - 1 and
+ * E4/E2 - 1, E2: true MSE of the estimated predictor, E3: true MSE of the
+ * optimal predictor, E4: MSE presumed under estimated_theta); data->status
+ * is nonzero when a covariance matrix is not positive definite.  The
+ * _Async variant is synchronous here. */
+void EXAGEOSTAT_dmle_mloe_mmom_Allocate(MLE_data *data, int nZmiss, int nZobs, int dts, int p_grid, int q_grid);
+void EXAGEOSTAT_dmle_mloe_mmom_Tile(MLE_data *data, double *truth_theta, double *estimated_theta, int nZmiss,
+                                    int nZobs, int n);
+void EXAGEOSTAT_dmle_mloe_mmom_Tile_Async(MLE_data *data, double *truth_theta, double *estimated_theta, int nZmiss,
+                                          int nZobs, int n);
+void mloe_mmom_init(MLE_data *data, int nZmiss, int nZobs, int ts, int p_grid, int q_grid);
+void mloe_mmom_finalize(MLE_data *data);
+void MLOE_MMOM_Finalize(MLE_data *data);
+/* No reference counterpart: the same computation returning the per-location
+ * values loe[j] = E2/E3 - 1 and mom[j] = E4/E2 - 1 (each array nZmiss, may be
+ * NULL).  Returns 0, > 0 when a covariance matrix is not positive definite,
+ * -1 on an invalid argument. */
+int exageostat_mloe_mmom(MLE_data *data, const double *truth_theta, const double *estimated_theta, int nZmiss,
+                         int nZobs, double *loe, double *mom);
 
 /* ------------------------------------------------- Fisher information -- */
 /* exageostat_exact/src/compute/MLE_dexact.c:1777-1934 (declared in

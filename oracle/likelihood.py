@@ -11,6 +11,10 @@ Pinning (tests/test_oracle_likelihood.py):
   (noise-free kriging interpolates) and matches an explicit
   C12 @ inv(C22) @ z.
 * ``gradient`` against central finite differences of ``dmle``.
+* ``mloe_mmom``: every per-location MSE against the quadratic form of the
+  joint (observed + held-out) covariance, the optimal MSE against the
+  Schur-complement identity 1 / (Sigma_joint^-1)_00, zero MLOE and MMOM at
+  the true theta, MLOE >= 0, and a Monte Carlo estimate of the true MSE.
 """
 from __future__ import annotations
 
@@ -117,6 +121,50 @@ def fisher(kernel: str, x, y, theta, metric: int = 0):
     Lf = np.linalg.cholesky(F)
     Finv = sla.cho_solve((Lf, True), np.eye(npar))
     return F, Finv
+
+
+def mloe_mmom(kernel: str, xobs, yobs, xmiss, ymiss, theta_true, theta_est, metric: int = 0):
+    """EXAGEOSTAT_dmle_mloe_mmom_Tile (MLE_dexact.c:1112-1435) for the
+    univariate kernels, location by location as the reference loops:
+    K_a, K_t = Sigma(lobs, lobs) under theta_est / theta_true, factored
+    (:1233-1249); per held-out location k_t, k_a = Sigma(lobs, s0)
+    (:1267-1270), k_a <- L_a^-T L_a^-1 k_a, k_t <- L_t^-T L_t^-1 k_t
+    (:1283-1311), expr1 = k_t0^T k_a, expr4 = k_a0^T k_a, expr3 = k_t0^T k_t
+    (:1313-1333), expr2 = |L_t^T k_a|^2 (:1336-1343), then
+    expr1 = s_t - 2 expr1, expr2 += expr1, expr3 = s_t - expr3,
+    expr4 = s_a - expr4 with s = theta[0] (:1349-1353, :1200-1203), and
+    loe = expr2 / expr3 - 1, mom = expr4 / expr2 - 1, each -1 when its
+    denominator is zero (codelet_zmloe_mmom.c:47-59).
+    Returns (mloe, mmom, loe, mom, parts) with parts = (expr2, expr3, expr4)
+    per location."""
+    Ka = matern.cov_matrix(kernel, xobs, yobs, xobs, yobs, theta_est, metric)
+    Kt = matern.cov_matrix(kernel, xobs, yobs, xobs, yobs, theta_true, metric)
+    La = np.linalg.cholesky(Ka)
+    Lt = np.linalg.cholesky(Kt)
+    st, sa = float(theta_true[0]), float(theta_est[0])
+    nm = len(xmiss)
+    loe, mom = np.zeros(nm), np.zeros(nm)
+    parts = np.zeros((3, nm))
+    for p in range(nm):
+        kt0 = matern.cov_matrix(kernel, xobs, yobs, xmiss[p:p + 1], ymiss[p:p + 1], theta_true, metric)[:, 0]
+        ka0 = matern.cov_matrix(kernel, xobs, yobs, xmiss[p:p + 1], ymiss[p:p + 1], theta_est, metric)[:, 0]
+        ka = sla.solve_triangular(La, ka0, lower=True)
+        kt = sla.solve_triangular(Lt, kt0, lower=True)
+        ka = sla.solve_triangular(La, ka, lower=True, trans="T")
+        kt = sla.solve_triangular(Lt, kt, lower=True, trans="T")
+        e1 = float(kt0 @ ka)
+        e4 = float(ka0 @ ka)
+        e3 = float(kt0 @ kt)
+        v = Lt.T @ ka
+        e2 = float(v @ v)
+        e1 = st - 2.0 * e1
+        e2 = e1 + e2
+        e3 = st - e3
+        e4 = sa - e4
+        loe[p] = -1.0 if e3 == 0.0 else e2 / e3 - 1.0
+        mom[p] = -1.0 if e2 == 0.0 else e4 / e2 - 1.0
+        parts[:, p] = (e2, e3, e4)
+    return float(np.mean(loe)), float(np.mean(mom)), loe, mom, parts
 
 
 def flops_dpotrf(n: float) -> float:
