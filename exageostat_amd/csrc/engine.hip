@@ -200,14 +200,31 @@ void DeviceVector::release() {
   n = padded = 0;
 }
 
+// FNV-1a over the 64-bit patterns of the coordinates.
+static uint64_t location_fingerprint(const double* x, const double* y, long n) {
+  uint64_t h = 1469598103934665603ull;
+  for (long i = 0; i < n; ++i) {
+    uint64_t a, b;
+    std::memcpy(&a, x + i, sizeof(a));
+    std::memcpy(&b, y + i, sizeof(b));
+    h = (h ^ a) * 1099511628211ull;
+    h = (h ^ b) * 1099511628211ull;
+  }
+  return h;
+}
+
 void DeviceLocations::upload(const double* x, const double* y, long n_, hipStream_t s) {
-  if (x == hx && y == hy && n_ == n && dx) return;
-  release();
-  n = n_;
+  const uint64_t f = location_fingerprint(x, y, n_);
+  if (x == hx && y == hy && n_ == n && dx && f == fp) return;
+  if (n_ != n || !dx) {
+    release();
+    n = n_;
+    EXA_HIP(hipMalloc(&dx, sizeof(double) * (n > 0 ? n : 1)));
+    EXA_HIP(hipMalloc(&dy, sizeof(double) * (n > 0 ? n : 1)));
+  }
   hx = x;
   hy = y;
-  EXA_HIP(hipMalloc(&dx, sizeof(double) * (n > 0 ? n : 1)));
-  EXA_HIP(hipMalloc(&dy, sizeof(double) * (n > 0 ? n : 1)));
+  fp = f;
   EXA_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
   EXA_HIP(hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, s));
   EXA_HIP(hipStreamSynchronize(s));
@@ -219,6 +236,7 @@ void DeviceLocations::release() {
   dx = dy = nullptr;
   hx = hy = nullptr;
   n = 0;
+  fp = 0;
 }
 
 // ------------------------------------------------------------- generation --

@@ -77,11 +77,15 @@ struct PhaseTimes {
   double gen = 0, facto = 0, logdet = 0, solve = 0, grad = 0, total = 0;
 };
 
-// Device copy of a location set (x, y), cached by host pointer.
+// Device copy of a location set (x, y).  upload() copies again only when
+// the host pointers, the count or a fingerprint of the host values differ
+// from the last upload, so unchanged locations stay resident across calls
+// and changed ones (also at the same address) are sent again.
 struct DeviceLocations {
   const double* hx = nullptr;
   const double* hy = nullptr;
   long n = 0;
+  uint64_t fp = 0;
   double* dx = nullptr;
   double* dy = nullptr;
   void upload(const double* x, const double* y, long n_, hipStream_t s);

@@ -35,8 +35,9 @@ struct MixedState {
   double* part = nullptr;  // partial sums of Sigma x: 2 * mt columns of mt*ts
   long part_len = 0;
   // Per-tile bounding boxes of the location set (xmin, xmax, ymin, ymax),
-  // cached for one host location array.
+  // cached for one host location array and its fingerprint.
   std::vector<double> bbox;
+  uint64_t bbox_fp = 0;
   const double* bbox_x = nullptr;
   const double* bbox_y = nullptr;
   long bbox_n = 0;
@@ -51,9 +52,10 @@ void destroy_mixed(MixedState* m) { delete m; }
 
 // Per-tile bounding boxes: tile t covers rows [t*ts, (t+1)*ts); row i sits at
 // location i / vars.
-static void tile_bboxes(MixedState& st, const double* x, const double* y, long nloc, int vars, long nrows, int ts,
-                        int mt) {
-  if (st.bbox_x == x && st.bbox_y == y && st.bbox_n == nloc && st.bbox_ts == ts && (long)st.bbox.size() == 4L * mt)
+static void tile_bboxes(MixedState& st, const double* x, const double* y, uint64_t fp, long nloc, int vars,
+                        long nrows, int ts, int mt) {
+  if (st.bbox_x == x && st.bbox_y == y && st.bbox_fp == fp && st.bbox_n == nloc && st.bbox_ts == ts &&
+      (long)st.bbox.size() == 4L * mt)
     return;
   st.bbox.assign(4L * mt, 0.0);
   for (int t = 0; t < mt; ++t) {
@@ -73,6 +75,7 @@ static void tile_bboxes(MixedState& st, const double* x, const double* y, long n
   }
   st.bbox_x = x;
   st.bbox_y = y;
+  st.bbox_fp = fp;
   st.bbox_n = nloc;
   st.bbox_ts = ts;
 }
@@ -109,7 +112,7 @@ static std::vector<uint8_t> precision_map(MixedState& st, const Context& ctx, Ke
     con.push_back(1.0 / (std::pow(2.0, nu - 1.0) * std::tgamma(nu)));
   }
   const bool boxes = metric == 0 && ctx.l1.hx && ctx.l1.hy;
-  if (boxes) tile_bboxes(st, ctx.l1.hx, ctx.l1.hy, ctx.l1.n, vars, ctx.N, ts, mt);
+  if (boxes) tile_bboxes(st, ctx.l1.hx, ctx.l1.hy, ctx.l1.fp, ctx.l1.n, vars, ctx.N, ts, mt);
   std::vector<uint8_t> prec((size_t)mt * (mt + 1) / 2);
   const int thick = std::max(1, opt.diag_thick);
   long idx = 0;

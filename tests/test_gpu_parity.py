@@ -175,6 +175,32 @@ def test_loglik_permutation_invariance(lib, c1):
     m.close()
 
 
+def test_locations_changed_in_place(lib, c1):
+    """Location values changed at the same host address between two calls
+    are sent to the device again (FP64 and mixed paths): l follows the
+    oracle at the new locations."""
+    x = np.array(c1["x"], dtype=np.float64)
+    y = np.array(c1["y"], dtype=np.float64)
+    th = c1["theta_true"]
+    m = lib.ExactMLE(x, y, 320)
+    m.set_z(c1["z"])
+    v1 = m.loglik(th)
+    m.precision = 2
+    m.set_mixed(fp32_corr=0.01, bf16=True)
+    w1 = m.loglik(th)
+    x *= 0.8  # in place: same buffer, new values
+    m.precision = 0
+    v2 = m.loglik(th)
+    m.precision = 2
+    w2 = m.loglik(th)
+    m.close()
+    ref1, _, _ = likelihood.dmle(K, c1["x"], c1["y"], c1["z"], th)
+    ref2, _, _ = likelihood.dmle(K, x, y, c1["z"], th)
+    assert abs(ref2 - ref1) > 1e-3 * abs(ref1)
+    assert abs(v1 - ref1) <= 1e-9 * abs(ref1) and abs(v2 - ref2) <= 1e-9 * abs(ref2), (v1, v2, ref1, ref2)
+    assert abs(w1 - ref1) <= 1e-6 * abs(ref1) and abs(w2 - ref2) <= 1e-6 * abs(ref2), (w1, w2, ref1, ref2)
+
+
 def test_loglik_non_spd_status(lib, c1):
     m = lib.ExactMLE(c1["x"], c1["y"], 320)
     m.set_z(c1["z"])
