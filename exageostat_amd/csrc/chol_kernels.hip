@@ -16,58 +16,75 @@ namespace exa {
 using C64 = GemmCfg<64, 64, 2, 2, 16>;
 
 // ---------------------------------------------------------------------------
-// Unblocked Cholesky of the 64x64 diagonal block j of a diagonal tile, in LDS,
-// plus the explicit inverse of the resulting lower-triangular block.
-__global__ void __launch_bounds__(256) k_potrf_block(double* T, int ts, int j, double* dinv, int* info,
-                                                     long col_base) {
+// Unblocked Cholesky of the 64x64 diagonal block j of a diagonal tile, plus
+// the explicit inverse of the resulting lower-triangular block, by one wave
+// (no multi-wave barriers on the critical path of the panel).  Lane r owns
+// row r.  Column c is finished left-looking,
+//   L(r,c) = (A(r,c) - sum_{k<c} L(r,k) L(c,k)) / L(c,c),
+// and lane r then forms column r of X = L^-1 by forward substitution,
+//   X(i,r) = (delta_ir - sum_{k<i} L(i,k) X(k,r)) / L(i,i);
+// each sum is taken as four interleaved partial sums (shorter dependent
+// chains).
+// Both matrices live in LDS column-major (element (i,k) at k*LD + i), so a
+// lane's own entries of one column are contiguous across the wave and
+// L(c,k) / L(i,k) are broadcast reads.
+__global__ void __launch_bounds__(64) k_potrf_block(double* T, int ts, int j, double* dinv, int* info,
+                                                    long col_base) {
   constexpr int LD = 65;
-  __shared__ double L[64 * LD];
-  __shared__ double Inv[64 * LD];
-  __shared__ double rdiag[64];
-  const int tid = threadIdx.x;
-  const int r = tid & 63;
-  const int g = tid >> 6;
+  __shared__ double Ls[64 * LD];  // L(i, k) at Ls[k * LD + i]
+  __shared__ double Xs[64 * LD];  // X(i, k) at Xs[k * LD + i] (written transposed, see below)
+  const int r = threadIdx.x;
   double* blk = T + (long)j * kNB * (ts + 1);
-  for (int c = g; c < 64; c += 4) L[c * LD + r] = blk[r + (long)c * ts];
+  {
+    // all 64 column loads in flight before the first LDS store
+    double col[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) col[k] = blk[r + (long)k * ts];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) Ls[k * LD + r] = col[k];
+  }
   __syncthreads();
   for (int c = 0; c < 64; ++c) {
-    double d = L[c * LD + c];
-    const bool bad = !(d > 0.0);
-    d = bad ? __builtin_nan("") : sqrt(d);
-    if (bad && tid == 0) atomicCAS(info, 0, (int)(col_base + c + 1));
-    __syncthreads();
-    if (tid < 64) {
-      if (tid > c) L[c * LD + tid] /= d;
-      if (tid == c) L[c * LD + c] = d;
+    // four partial sums over k mod 4 break the dependent FMA chain
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    int k = 0;
+#pragma unroll 4
+    for (; k + 4 <= c; k += 4) {
+      p0 += Ls[k * LD + r] * Ls[k * LD + c];
+      p1 += Ls[(k + 1) * LD + r] * Ls[(k + 1) * LD + c];
+      p2 += Ls[(k + 2) * LD + r] * Ls[(k + 2) * LD + c];
+      p3 += Ls[(k + 3) * LD + r] * Ls[(k + 3) * LD + c];
+    }
+    for (; k < c; ++k) p0 += Ls[k * LD + r] * Ls[k * LD + c];
+    const double s = Ls[c * LD + r] - ((p0 + p1) + (p2 + p3));
+    if (r == c) {
+      const bool bad = !(s > 0.0);
+      if (bad) atomicCAS(info, 0, (int)(col_base + c + 1));
+      Ls[c * LD + c] = bad ? __builtin_nan("") : sqrt(s);
     }
     __syncthreads();
-    for (int cc = c + 1 + g; cc < 64; cc += 4) {
-      if (r >= cc) L[cc * LD + r] -= L[c * LD + r] * L[c * LD + cc];
-    }
+    if (r > c) Ls[c * LD + r] = s / Ls[c * LD + c];
     __syncthreads();
   }
-  if (tid < 64) rdiag[tid] = 1.0 / L[tid * LD + tid];
-  // X = L^-1 by row sweeps: X starts as I; at step rr row rr is final after
-  // scaling by 1/L[rr][rr], then X[i][c] -= L[i][rr] X[rr][c] for i > rr,
-  // c <= rr (all threads in parallel).
-  for (int c = g; c < 64; c += 4) Inv[c * LD + r] = (r == c) ? 1.0 : 0.0;
+  for (int k = 0; k < 64; ++k) blk[r + (long)k * ts] = (r >= k) ? Ls[k * LD + r] : 0.0;
+  // Lane r keeps X(i, r) at Xs[i * LD + r] (its own column, contiguous across
+  // lanes for each i).
+  for (int i = 0; i < 64; ++i) {
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    int k = 0;
+#pragma unroll 4
+    for (; k + 4 <= i; k += 4) {
+      p0 += Ls[k * LD + i] * Xs[k * LD + r];
+      p1 += Ls[(k + 1) * LD + i] * Xs[(k + 1) * LD + r];
+      p2 += Ls[(k + 2) * LD + i] * Xs[(k + 2) * LD + r];
+      p3 += Ls[(k + 3) * LD + i] * Xs[(k + 3) * LD + r];
+    }
+    for (; k < i; ++k) p0 += Ls[k * LD + i] * Xs[k * LD + r];
+    Xs[i * LD + r] = (((i == r) ? 1.0 : 0.0) - ((p0 + p1) + (p2 + p3))) / Ls[i * LD + i];
+  }
   __syncthreads();
-  for (int rr = 0; rr < 64; ++rr) {
-    if (tid <= rr) Inv[tid * LD + rr] *= rdiag[rr];
-    __syncthreads();
-    const int w = rr + 1;
-    const int cnt = (63 - rr) * w;
-    for (int e = tid; e < cnt; e += 256) {
-      const int i = rr + 1 + e / w;
-      const int c = e - (e / w) * w;
-      Inv[c * LD + i] -= L[rr * LD + i] * Inv[c * LD + rr];
-    }
-    __syncthreads();
-  }
-  for (int c = g; c < 64; c += 4) {
-    blk[r + (long)c * ts] = (r >= c) ? L[c * LD + r] : 0.0;
-    dinv[r + c * 64] = Inv[c * LD + r];
-  }
+  // dinv column-major: dinv[i + c*64] = X(i, c) = Xs[i * LD + c].
+  for (int c = 0; c < 64; ++c) dinv[r + c * 64] = Xs[r * LD + c];
 }
 
 // ---------------------------------------------------------------------------
@@ -374,7 +391,7 @@ void launch_tile_update(const TileView& v, int k, int np, int col_lo, int col_hi
 void launch_potrf_tile(double* T, int ts, double* dinv_k, int* info, long col_base, hipStream_t s) {
   const int q = ts / kNB;
   for (int j = 0; j < q; ++j) {
-    hipLaunchKernelGGL(k_potrf_block, dim3(1), dim3(256), 0, s, T, ts, j, dinv_k + (long)j * 4096, info,
+    hipLaunchKernelGGL(k_potrf_block, dim3(1), dim3(64), 0, s, T, ts, j, dinv_k + (long)j * 4096, info,
                        col_base + (long)j * kNB);
     if (j + 1 < q) {
       const int nb = q - j - 1;
