@@ -375,6 +375,8 @@ def main() -> None:
         "solve_check_rel": abs(f64["dotp"] - ee) / ee,
         "roofline": roof64,
         "gen_GBps": (_rate(f64["prof"]["gen"], "bytes")[0] or 0) / 1e9,
+        # lower-triangle entries generated per second (8 bytes each)
+        "gen_Gentries_per_s": (_rate(f64["prof"]["gen"], "bytes")[0] or 0) / 8e9,
         "trsv_GBps": (_rate(f64["prof"]["trsv"], "bytes")[0] or 0) / 1e9,
     }
     if mix is None:
