@@ -16,7 +16,8 @@ and RNG work), what the reference computes on this path:
   core_dcmg_bivariate_parsimonious and the Matern derivative kernels
   (exageostat_exact/cpu_core/compute/core_zcmg.c, core_zcmg_nuggets.c).
 * ``likelihood`` -- EXAGEOSTAT_dmle_Tile, EXAGEOSTAT_MLE_dzvg_Tile,
-  EXAGEOSTAT_dmle_Predict_Tile and the dense adjoint gradient
+  EXAGEOSTAT_dmle_Predict_Tile, the dense adjoint gradient,
+  EXAGEOSTAT_Fisher_Tile and EXAGEOSTAT_dmle_mloe_mmom_Tile
   (exageostat_exact/src/compute/MLE_dexact.c).
 
 How each function is pinned (golden vectors, library cross-checks,
