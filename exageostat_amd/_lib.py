@@ -35,31 +35,47 @@ class Desc(ctypes.Structure):
     ]
 
 
+_D, _I, _F = ctypes.c_double, ctypes.c_int, ctypes.c_float
+_S, _V, _DP = ctypes.c_char_p, ctypes.c_void_p, c_double_p
+
+
 class MLEData(ctypes.Structure):
+    """MLE_data: the reference's fields in its order (FILE* as void*), then
+    this implementation's fields; `async_` is the C field `async`."""
     _fields_ = [
-        ("variance", ctypes.c_double), ("variance1", ctypes.c_double), ("variance2", ctypes.c_double),
-        ("computation", ctypes.c_char_p), ("test", ctypes.c_int), ("async_", ctypes.c_int),
-        ("iter_count", ctypes.c_int), ("l1", Location), ("lmiss", Location), ("lobs", Location),
-        ("lm", Location), ("descC", ctypes.c_void_p), ("descZ", ctypes.c_void_p),
-        ("descZcpy", ctypes.c_void_p), ("descdet", ctypes.c_void_p), ("descproduct", ctypes.c_void_p),
-        ("descZmiss", ctypes.c_void_p), ("descC12", ctypes.c_void_p), ("descC22", ctypes.c_void_p),
-        ("descZactual", ctypes.c_void_p), ("descZobs", ctypes.c_void_p), ("descmse", ctypes.c_void_p),
-        ("sequence", ctypes.c_void_p), ("request", ctypes.c_void_p), ("verbose", ctypes.c_int),
-        ("log", ctypes.c_int), ("avg_exec_time_per_iter", ctypes.c_double),
-        ("total_exec_time", ctypes.c_double), ("avg_flops_per_iter", ctypes.c_double),
-        ("avg_generation_per_iter", ctypes.c_double), ("avg_solve_per_iter", ctypes.c_double),
-        ("avg_cholesky_per_iter", ctypes.c_double), ("final_loglik", ctypes.c_double),
-        ("recovery_file", ctypes.c_char_p), ("checkpoint_file", ctypes.c_char_p),
-        ("det", ctypes.c_double), ("dotp", ctypes.c_double), ("mserror", ctypes.c_double),
-        ("mserror1", ctypes.c_double), ("mserror2", ctypes.c_double), ("dm", ctypes.c_char_p),
-        ("diag_thick", ctypes.c_int), ("opt_tol", ctypes.c_int), ("opt_max_iters", ctypes.c_int),
-        ("kernel_fun", ctypes.c_char_p), ("precision", ctypes.c_int), ("mspe", ctypes.c_int),
-        ("seed", ctypes.c_int), ("mloe", ctypes.c_double), ("mmom", ctypes.c_double),
-        ("gpu", ctypes.c_void_p), ("silent", ctypes.c_int),
-        ("status", ctypes.c_int), ("logdet", ctypes.c_double), ("time_gen", ctypes.c_double),
-        ("time_facto", ctypes.c_double), ("time_logdet", ctypes.c_double), ("time_solve", ctypes.c_double),
-        ("time_total", ctypes.c_double), ("mp_fp32_corr", ctypes.c_double), ("mp_bf16", ctypes.c_int),
-        ("mp_ir_steps", ctypes.c_int), ("mp_fp64_corr", ctypes.c_double), ("mp_lolo", ctypes.c_int),
+        ("variance", _D), ("variance1", _D), ("variance2", _D), ("variance3", _D), ("computation", _S),
+        ("c_fun", _S), ("test", _I), ("async_", _I), ("iter_count", _I), ("l1", Location), ("lmiss", Location),
+        ("lobs", Location), ("lm", Location), ("descC", _V), ("descsubC11", _V), ("descsubC12", _V),
+        ("descsubC21", _V), ("descsubC22", _V), ("descZ", _V), ("descZ1", _V), ("descZ2", _V), ("descZ3", _V),
+        ("Adense", _DP), ("Adense2", _DP), ("descZcpy", _V), ("descdet", _V), ("descsum", _V),
+        ("descproduct", _V), ("descproduct1", _V), ("descproduct2", _V), ("descproduct3", _V),
+        ("descZmiss", _V), ("descC11", _V), ("descC21", _V), ("descC12", _V), ("descC22", _V),
+        ("descZactual", _V), ("descZobs", _V), ("descmse", _V), ("descmse1", _V), ("descmse2", _V),
+        ("descmse3", _V), ("descZtrace", _V), ("sequence", _V), ("request", _V), ("verbose", _I),
+        ("check", _I), ("log", _I), ("avg_exec_time_per_iter", _D), ("total_exec_time", _D),
+        ("avg_flops_per_iter", _D), ("avg_generation_per_iter", _D), ("avg_solve_per_iter", _D),
+        ("avg_cholesky_per_iter", _D), ("avg_exec_time_gen_stage", _D), ("avg_flops_gen_stage", _D),
+        ("final_loglik", _D), ("locsFPath", _S), ("timeFPath", _S), ("obsFPath", _S), ("obsFPath2", _S),
+        ("obsFPath3", _S), ("actualZFPath", _S), ("actualZFPath2", _S), ("actualZFPath3", _S),
+        ("actualZLocFPath", _S), ("actualTimeFPath", _S), ("det", _D), ("sum", _D), ("dotp", _D),
+        ("dotp1", _D), ("dotp2", _D), ("dotp3", _D), ("sdotp", _F), ("mserror", _D), ("mserror1", _D),
+        ("mserror2", _D), ("mserror3", _D), ("dm", _S), ("diag_thick", _I), ("nFileLog", _S), ("pFileLog", _V),
+        ("hicma_maxrank", _I), ("hicma_data_type", _I), ("hicma_descC", _V), ("hicma_descZ", _V),
+        ("hicma_descCD", _V), ("hicma_descCUV", _V), ("hicma_descCrk", _V), ("hicma_descZcpy", _V),
+        ("hicma_descdet", _V), ("hicma_descproduct", _V), ("hicma_descC12D", _V), ("hicma_descC12UV", _V),
+        ("hicma_descC12rk", _V), ("hicma_descC22D", _V), ("hicma_descC22UV", _V), ("hicma_descC22rk", _V),
+        ("hicma_acc", _D), ("hsequence", _V), ("hrequest", _V), ("opt_tol", _I), ("opt_max_iters", _I),
+        ("ooc", _I), ("kernel_fun", _S), ("precision", _I), ("desctemp", _V), ("desctemp22", _V),
+        ("desck_t", _V), ("desck_a", _V), ("desck_ttmp", _V), ("desck_atmp", _V), ("descK_t", _V),
+        ("descK_a", _V), ("descexpr1", _V), ("descexpr2", _V), ("descexpr3", _V), ("descexpr4", _V),
+        ("descestimatedalpha", _V), ("desctruthalpha", _V), ("desc_mloe_mmom", _V), ("descmloe", _V),
+        ("descmmom", _V), ("expr1", _D), ("expr2", _D), ("expr3", _D), ("expr4", _D), ("mloe", _D),
+        ("mmom", _D), ("mloe_mmom", _I), ("mloe_mmom_async", _I), ("mspe", _I), ("recovery_file", _S),
+        ("checkpoint_file", _S), ("time_slots", _I), ("idw", _I), ("fisher", _I), ("descr", _V),
+        ("descrcpy", _V), ("seed", _I), ("gpu", _V), ("silent", _I), ("status", _I), ("logdet", _D),
+        ("time_gen", _D), ("time_facto", _D), ("time_logdet", _D), ("time_solve", _D), ("time_total", _D),
+        ("mp_fp32_corr", _D), ("mp_bf16", _I), ("mp_ir_steps", _I), ("mp_fp64_corr", _D), ("mp_lolo", _I),
+        ("mp_ir_tol", _D), ("mp_ir_done", _I), ("mp_ir_change", _D),
     ]
 
 
@@ -78,6 +94,23 @@ _SIGNATURES = {
     "EXAGEOSTAT_dmle_Tile": (ctypes.c_double, [ctypes.c_uint, c_double_p, c_double_p, ctypes.c_void_p]),
     "MORSE_dmle_Tile": (ctypes.c_double, [ctypes.c_uint, c_double_p, c_double_p, ctypes.c_void_p]),
     "EXAGEOSTAT_sdmle_Tile": (ctypes.c_double, [ctypes.c_uint, c_double_p, c_double_p, ctypes.c_void_p]),
+    "EXAGEOSTAT_dmle_Tile_Async": (ctypes.c_double, [ctypes.c_uint, c_double_p, c_double_p, ctypes.c_void_p]),
+    "EXAGEOSTAT_sdmle_Tile_Async": (ctypes.c_double, [ctypes.c_uint, c_double_p, c_double_p, ctypes.c_void_p]),
+    "EXAGEOSTAT_smle_Tile": (ctypes.c_double, [ctypes.c_uint, c_double_p, c_double_p, ctypes.c_void_p]),
+    "EXAGEOSTAT_smle_Tile_Async": (ctypes.c_double, [ctypes.c_uint, c_double_p, c_double_p, ctypes.c_void_p]),
+    "EXAGEOSTAT_dmle_diag_Tile": (ctypes.c_double, [ctypes.c_uint, c_double_p, c_double_p, ctypes.c_void_p]),
+    "EXAGEOSTAT_dmle_diag_Tile_Async": (ctypes.c_double, [ctypes.c_uint, c_double_p, c_double_p, ctypes.c_void_p]),
+    "EXAGEOSTAT_smle_Call": (None, [_P(MLEData)] + [ctypes.c_int] * 8),
+    "EXAGEOSTAT_sdmle_Call": (None, [_P(MLEData)] + [ctypes.c_int] * 8),
+    "EXAGEOSTAT_dmle_diag_Call": (None, [_P(MLEData)] + [ctypes.c_int] * 8),
+    "EXAGEOSTAT_MLE_dzvg_Tile_Async": (None, [_P(MLEData), c_double_p, c_double_p] + [ctypes.c_int] * 3),
+    "EXAGEOSTAT_dmle_Predict_Tile_Async": (ctypes.c_double, [_P(MLEData), c_double_p] + [ctypes.c_int] * 3),
+    "EXAGEOSTAT_smle_Predict_Tile": (ctypes.c_double, [_P(MLEData), c_double_p, ctypes.c_int, ctypes.c_int,
+                                                       c_double_p, c_double_p, c_double_p, ctypes.c_int]),
+    "EXAGEOSTAT_sdmle_Predict_Tile": (ctypes.c_double, [_P(MLEData), c_double_p, ctypes.c_int, ctypes.c_int,
+                                                        c_double_p, c_double_p, c_double_p, ctypes.c_int]),
+    "EXAGEOSTAT_dmle_diag_Predict_Tile": (ctypes.c_double, [_P(MLEData), c_double_p, ctypes.c_int, ctypes.c_int,
+                                                            c_double_p, c_double_p, c_double_p, ctypes.c_int]),
     "EXAGEOSTAT_MLE_dcmg_Tile_Async": (ctypes.c_int, [ctypes.c_int, _P(Desc), _P(Location), _P(Location),
                                                       _P(Location), c_double_p, ctypes.c_char_p,
                                                       ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p]),

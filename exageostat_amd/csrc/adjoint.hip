@@ -54,7 +54,7 @@ void destroy_adjoint(AdjointState* a) { delete a; }
 
 // Z = Sigma^-1 in place of the factor held in A (tile_potrf output, with the
 // inverted diagonal tiles in ws).
-void tile_potri_inplace(TileMatrix& A, const CholWorkspace& ws, double* P, double* Q, hipStream_t s) {
+void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, hipStream_t s) {
   if (ws.factored != A.tile_raw(0, 0) || ws.mt_ < A.mt || ws.ts_ != A.ts)
     throw std::invalid_argument("potri: the matrix is not the last one factored");
   const TileView v = A.view();
@@ -72,6 +72,7 @@ void tile_potri_inplace(TileMatrix& A, const CholWorkspace& ws, double* P, doubl
     profiled("potri_diag", 2.0 * below * rk * rk + rk * rk * rk / 3.0, 0.0, s,
              [&] { launch_takahashi_stage(v, k, ws.xinv(k), P, Q, A.tile(k, k), 2, s); });
   }
+  ws.factored = nullptr;
 }
 
 // Scratch of the Takahashi sweep and of the dSigma reductions, sized for

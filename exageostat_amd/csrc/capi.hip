@@ -167,14 +167,59 @@ void copy_vec_to_device(DeviceVector& v, const double* h, long n, hipStream_t s)
   EXA_HIP(hipStreamSynchronize(s));
 }
 
+// The reference's per-iteration lines (MLE_dexact.c:439-472), also to
+// data->pFileLog when data->log == 1.
 void print_iteration(MLE_data* data, const double* theta, int np, double loglik) {
-  if (data->silent) return;
-  fprintf(stderr, " %3d- Model Parameters (", data->iter_count + 1);
-  for (int i = 0; i < np; ++i) fprintf(stderr, "%.8f%s", theta[i], i + 1 < np ? "," : "");
-  fprintf(stderr, ")----> LogLi: %.18f\n", loglik);
-  fprintf(stderr, " ---- Facto Time: %6.2f\n", data->time_facto);
-  fprintf(stderr, " ---- Matrix Generation Time: %6.2f\n", data->time_gen);
-  fprintf(stderr, " ---- Total Time: %6.2f\n", data->time_total);
+  FILE* logf = (data->log == 1) ? data->pFileLog : nullptr;
+  if (data->silent && !logf) return;
+  for (FILE* f : {data->silent ? (FILE*)nullptr : stderr, logf}) {
+    if (!f) continue;
+    fprintf(f, " %3d- Model Parameters (", data->iter_count + 1);
+    for (int i = 0; i < np; ++i) fprintf(f, "%.8f%s", theta[i], i + 1 < np ? "," : "");
+    fprintf(f, ")----> LogLi: %.18f\n", loglik);
+    if (f == logf) continue;
+    fprintf(f, " ---- Facto Time: %6.2f\n", data->time_facto);
+    fprintf(f, " ---- Matrix Generation Time: %6.2f\n", data->time_gen);
+    fprintf(f, " ---- Total Time: %6.2f\n", data->time_total);
+  }
+}
+
+// recover() of misc/compute/MLE_misc.c:2035-2070: the line of `path` whose
+// first field is iter_count holds theta (np values) and the likelihood.
+// Returns true and fills theta_out / loglik when the line exists.
+bool recover_iteration(const char* path, int iter_count, int np, std::vector<double>& theta_out, double& loglik) {
+  FILE* fp = fopen(path, "r");
+  if (!fp) throw std::runtime_error(std::string("cannot open the recovery file ") + path);
+  char* line = nullptr;
+  size_t cap = 0;
+  bool found = false;
+  while (!found && getline(&line, &cap, fp) != -1) {
+    char* save = nullptr;
+    char* tok = strtok_r(line, " \t\n", &save);
+    if (!tok || atoi(tok) != iter_count) continue;
+    theta_out.assign(np, 0.0);
+    int got = 0;
+    for (; got < np && (tok = strtok_r(nullptr, " \t\n", &save)); ++got) theta_out[got] = atof(tok);
+    tok = strtok_r(nullptr, " \t\n", &save);
+    if (got == np && tok) {
+      loglik = atof(tok);
+      found = true;
+    }
+  }
+  free(line);
+  fclose(fp);
+  return found;
+}
+
+// checkpointing() of misc/compute/MLE_misc.c:2014-2033: one line
+// "iter theta... loglik" appended per evaluation (the format recover reads).
+void checkpoint_iteration(const char* path, int iter_count, const double* theta, int np, double loglik) {
+  FILE* fp = fopen(path, "a");
+  if (!fp) throw std::runtime_error(std::string("cannot open the checkpoint file ") + path);
+  fprintf(fp, "%d ", iter_count);
+  for (int i = 0; i < np; ++i) fprintf(fp, "%.17g ", theta[i]);
+  fprintf(fp, "%.17g \n", loglik);
+  fclose(fp);
 }
 
 // Morton order of locations_obs_zsort_inplace (misc/compute/MLE_misc.c:
@@ -253,9 +298,228 @@ auto guarded(F&& f, decltype(f()) on_error) -> decltype(f()) {
   return on_error;
 }
 
+void release_prediction_descs(MLE_data* data) {
+  free_desc(data->descC22);
+  free_desc(data->descC12);
+  free_desc(data->descZobs);
+  free_desc(data->descZmiss);
+  free_desc(data->descZactual);
+  free_desc(data->descmse);
+  free_desc(data->descmse1);
+  free_desc(data->descmse2);
+}
+
+// Kriging (MLE_dexact.c:688-874): C22 = Sigma(lobs, lobs) and C12 =
+// Sigma(lmiss, lobs) generated on the device, C22 factored, alpha =
+// C22^-1 zobs by two triangular solves, zmiss = C12 alpha into ctx->Zmiss.
+// zobs: host values, or null to use ctx->Zobs as the caller filled it.
+void predict_core(MLE_data* data, Context* ctx, KernelId kid, const double* theta, int nZmiss, int nZobs,
+                  const location& lmiss, const location& lobs, const double* zobs) {
+  const int vars = kernel_vars(kid);
+  if (!ctx->C22.allocated() || ctx->C22.m != nZobs || ctx->C12.m != nZmiss) {
+    EXAGEOSTAT_dmle_Predict_Allocate(data, nZmiss / vars, nZobs / vars, ctx->ts, 1, 1, 1);
+    if (g_status < 0) throw std::runtime_error(g_error);
+  }
+  const long nobs_loc = nZobs / vars;
+  const long nmiss_loc = nZmiss / vars;
+  hipStream_t s = ctx->s1;
+  ctx->lobs.upload(lobs.x, lobs.y, nobs_loc, s, lobs.z);
+  ctx->lmiss.upload(lmiss.x, lmiss.y, nmiss_loc, s, lmiss.z);
+  const double t0 = wall_seconds();
+  generate_covariance(ctx->C22, kid, theta, ctx->lobs, ctx->lobs, metric_of(data->dm), s);
+  generate_covariance(ctx->C12, kid, theta, ctx->lmiss, ctx->lobs, metric_of(data->dm), s);
+  if (zobs) copy_vec_to_device(ctx->Zobs, zobs, nZobs, s);
+  const int info = tile_potrf(ctx->C22, ctx->ws, s, ctx->s2, true);
+  if (info != 0) throw std::runtime_error("prediction: C22 is not positive definite");
+  tile_trsv(ctx->C22, ctx->ws, ctx->Zobs.d, 0, s);
+  tile_trsv(ctx->C22, ctx->ws, ctx->Zobs.d, 1, s);
+  launch_gemv_full(ctx->C12.view(), ctx->Zobs.d, ctx->Zmiss.d, ctx->gemv_part.d, s);
+  EXA_HIP(hipStreamSynchronize(s));
+  const double t1 = wall_seconds();
+  ctx->last.total = t1 - t0;
+  data->time_total = t1 - t0;
+  results.total_pred_time = t1 - t0;
+}
+
+// MSE (codelet_zmse.c:22-38) and, for the bivariate kernels, per variable
+// (codelet_zmse_bivariate.c:22-51: by row parity; the block layout of
+// "bivariate_matern_parsimonious2" splits the rows into halves instead).
+double prediction_mse(MLE_data* data, KernelId kid, int vars, const double* Zactual, const double* Zmiss,
+                      int nZmiss) {
+  double se = 0.0, se1 = 0.0, se2 = 0.0;
+  for (int i = 0; i < nZmiss; ++i) {
+    const double e = (Zactual[i] - Zmiss[i]) * (Zactual[i] - Zmiss[i]);
+    se += e;
+    const bool second = kid == kBivariateBlock ? (i >= nZmiss / 2) : (i % 2 == 1);
+    if (second) se2 += e; else se1 += e;
+  }
+  data->mserror = se / nZmiss;
+  data->mserror1 = data->mserror2 = 0.0;
+  if (vars == 2) {
+    data->mserror1 = se1 / (nZmiss / 2);
+    data->mserror2 = se2 / (nZmiss / 2);
+  }
+  results.mse_pred = data->mserror;
+  results.mse_pred1 = data->mserror1;
+  results.mse_pred2 = data->mserror2;
+  return data->mserror;
+}
+
+// Likelihood paths of MLE_alg (src/compute/MLE.c:138-193).
+enum class Path {
+  kExact,   // precision 0, "exact": EXAGEOSTAT_dmle_Tile (MLE_dexact.c:231-489)
+  kMixed,   // precision 2: EXAGEOSTAT_sdmle_Tile (MLE_sdexact.c:172-357)
+  kSingle,  // precision 1: EXAGEOSTAT_smle_Tile (MLE_sexact.c:155-277)
+  kBand,    // "diag_approx": EXAGEOSTAT_dmle_diag_Tile (MLE_approx.c:201-365)
+};
+
+// One likelihood evaluation, shared by the entry points of every path: the
+// bookkeeping of the reference's evaluation functions (z restore, recovery
+// replay, iteration line, averages, results) around the engine call.
+double evaluate(MLE_data* data, unsigned n, const double* theta, double* grad, Path path) {
+  const double out = guarded([&]() {
+    Context* ctx = ctx_of(data);
+    const KernelId kid = kernel_from_name(data->kernel_fun);
+    if (kid == kUnknownKernel)
+      throw std::invalid_argument(std::string("unsupported kernel: ") + (data->kernel_fun ? data->kernel_fun : "(null)"));
+    const bool profile = kernel_is_profile(data->kernel_fun);
+    const int np = n ? (int)n : kernel_num_params(kid);
+    const int metric = metric_of(data->dm);
+    const double* lz = (kid == kSpaceTime || data->l1.z) ? data->l1.z : nullptr;
+    if (grad) {
+      // The mixed and single factors carry no Sigma^-1 accurate enough for
+      // the trace term; a gradient-based optimiser must not iterate on a
+      // made-up value, so those calls fail and say why.
+      if (path != Path::kExact)
+        throw std::invalid_argument(
+            "only precision 0 with computation \"exact\" computes a gradient: pass grad = NULL for precision "
+            "1 or 2 and for diag_approx");
+      if (!(kid == kUnivariate || kid == kNugget || kid == kBivariate) || profile || lz)
+        throw std::invalid_argument(
+            "the adjoint gradient covers the 2-D univariate Matern, nugget and interleaved parsimonious bivariate "
+            "kernels");
+    }
+    const long nloc = ctx->N / kernel_vars(kid);
+    ctx->l1.upload(data->l1.x, data->l1.y, nloc, ctx->s1, lz);
+    if (data->iter_count == 0) {
+      // Keep a pristine copy of Z for later iterations (MLE_dexact.c:303-317).
+      EXA_HIP(hipMemcpyAsync(ctx->Zcpy.d, ctx->Z.d, sizeof(double) * ctx->Z.padded, hipMemcpyDeviceToDevice,
+                             ctx->s1));
+      EXA_HIP(hipStreamSynchronize(ctx->s1));
+    }
+    double loglik = 0.0;
+    bool replayed = false;
+    std::vector<double> rtheta;
+    if (!grad && data->recovery_file && data->recovery_file[0])
+      replayed = recover_iteration(data->recovery_file, data->iter_count, np, rtheta, loglik);
+    int info = 0;
+    if (!replayed) {
+      double logdet = 0.0, dotp = 0.0;
+      MixedReport rep;
+      if (path == Path::kExact || path == Path::kBand) {
+        const int band = path == Path::kBand ? std::max(1, data->diag_thick) : 0;
+        if (grad) loglik = adjoint_loglik_grad(*ctx, kid, theta, metric, grad, &logdet, &dotp, &info);
+        else loglik = ctx->loglik(kid, theta, metric, &logdet, &dotp, &info, band);
+      } else {
+        MixedOptions opt;
+        if (path == Path::kMixed) {
+          opt.fp64_corr = data->mp_fp64_corr;
+          opt.fp32_corr = data->mp_fp32_corr;
+          opt.bf16 = data->mp_bf16 != 0;
+          opt.ir_steps = data->mp_ir_steps;
+          opt.ir_tol = data->mp_ir_tol;
+          opt.diag_thick = data->diag_thick;
+          opt.lolo = data->mp_lolo != 0;
+        } else {
+          // single precision: every off-diagonal tile FP32, no refinement
+          opt.ir_steps = 0;
+        }
+        loglik = mixed_loglik(*ctx, kid, theta, metric, opt, &logdet, &dotp, &info, &rep);
+        data->mp_ir_done = rep.ir_done;
+        data->mp_ir_change = rep.ir_change;
+        if (path == Path::kSingle) data->sdotp = (float)dotp;
+      }
+      const long N = ctx->N;
+      if (profile && info == 0) {
+        // Profile likelihood of the "_profile" kernels (MLE_dexact.c:403-421),
+        // N/2 in integer arithmetic as there; variance_v = |(L^-1 z)_v|^2 / (N/2).
+        const long half = N / 2;
+        loglik = -(double)half + (double)half * std::log((double)N) - (double)half * std::log(dotp) -
+                 0.5 * logdet - (N / 2.0) * std::log(2.0 * M_PI);
+        data->dotp1 = data->dotp2 = std::nan("");
+        if (path == Path::kExact || path == Path::kBand) {
+          std::vector<double> w(N);
+          EXA_HIP(hipMemcpyAsync(w.data(), ctx->work.d, sizeof(double) * N, hipMemcpyDeviceToHost, ctx->s1));
+          EXA_HIP(hipStreamSynchronize(ctx->s1));
+          double d1 = 0.0, d2 = 0.0;
+          for (long i = 0; i < N; ++i) {
+            const bool second = (kid == kBivariateBlock) ? (i >= half) : (i & 1);
+            (second ? d2 : d1) += w[i] * w[i];
+          }
+          data->dotp1 = d1;
+          data->dotp2 = d2;
+        }
+        data->variance1 = data->dotp1 / (double)half;
+        data->variance2 = data->dotp2 / (double)half;
+      }
+      data->det = 0.5 * logdet;
+      data->dotp = dotp;
+      data->logdet = logdet;
+      data->time_gen = ctx->last.gen;
+      data->time_facto = ctx->last.facto;
+      data->time_logdet = ctx->last.logdet;
+      data->time_solve = ctx->last.solve;
+      data->time_total = ctx->last.total;
+      data->status = info;
+      if (info != 0) {
+        set_error(info, "covariance matrix is not positive definite");
+        loglik = -HUGE_VAL;
+      } else if (!rep.ir_converged) {
+        // l is returned, but the caller learns that the quadratic form did
+        // not reach the requested accuracy.
+        set_error(-3, "mixed precision: the iterative refinement did not reach mp_ir_tol in mp_ir_steps steps");
+        data->status = -3;
+      }
+      data->avg_exec_time_per_iter += ctx->last.facto + ctx->last.logdet + ctx->last.solve;
+      const double flops = (double)N * (((1.0 / 6.0) * N + 0.5) * N + 1.0 / 3.0) +
+                           (double)N * (((1.0 / 6.0) * N) * N - 1.0 / 6.0) + (double)N * (N + 1.0);
+      if (ctx->last.facto + ctx->last.solve > 0)
+        data->avg_flops_per_iter += flops / 1e9 / (ctx->last.facto + ctx->last.solve);
+      data->avg_generation_per_iter += ctx->last.gen;
+      data->avg_cholesky_per_iter += ctx->last.facto;
+      data->avg_solve_per_iter += ctx->last.solve;
+    } else {
+      data->status = 0;
+    }
+    print_iteration(data, theta, np, loglik);
+    if (!replayed && data->checkpoint_file && data->checkpoint_file[0])
+      checkpoint_iteration(data->checkpoint_file, data->iter_count, theta, np, loglik);
+    data->variance = theta[0];
+    data->iter_count++;
+    data->final_loglik = loglik;
+    results.final_loglik = loglik;
+    if (results.estimated_theta) {
+      int i = 0;
+      if (profile) {
+        results.estimated_theta[0] = data->variance1;
+        results.estimated_theta[1] = data->variance2;
+        i = 2;
+      }
+      for (; i < np; ++i) results.estimated_theta[i] = theta[i];
+    }
+    if (info != 0) g_status = info;
+    else if (data->status == -3) g_status = -3;
+    return loglik;
+  }, -HUGE_VAL);
+  if (g_status < 0 && g_status != -3 && data) data->status = g_status;
+  return out;
+}
+
 }  // namespace
 
 extern "C" {
+
+output results;
 
 int exageostat_gpu_last_status(void) { return g_status; }
 const char* exageostat_gpu_last_error(void) { return g_error.c_str(); }
@@ -454,11 +718,11 @@ int EXAGEOSTAT_MLE_dcmg_Tile_Async(CHAM_enum uplo, CHAM_desc_t* descA, location*
     if (kid == kUnknownKernel) throw std::invalid_argument(std::string("unsupported kernel: ") + (kernel_fun ? kernel_fun : "(null)"));
     auto* impl = static_cast<DescImpl*>(descA->mat);
     TileMatrix& tm = *impl->tm;
-    const long nr = kid == kBivariate ? tm.m / 2 : tm.m;
-    const long nc = kid == kBivariate ? tm.n / 2 : tm.n;
+    const long nr = tm.m / kernel_vars(kid);
+    const long nc = tm.n / kernel_vars(kid);
     hipStream_t s = seq_stream(sequence);
-    impl->rows.upload(l1->x, l1->y, nr, s);
-    impl->cols.upload(l2->x, l2->y, nc, s);
+    impl->rows.upload(l1->x, l1->y, nr, s, l1->z);
+    impl->cols.upload(l2->x, l2->y, nc, s, l2->z);
     generate_covariance(tm, kid, theta, impl->rows, impl->cols, metric_of(dm), s);
     return 0;
   }, -1);
@@ -535,9 +799,28 @@ void EXAGEOSTAT_dmle_Call(MLE_data* data, int ncores, int gpus, int dts, int p_g
   }, 0);
 }
 
+// The single-precision, mixed-precision and band-limited paths share the
+// device context of EXAGEOSTAT_dmle_Call: the tile precision of each
+// evaluation is chosen when it runs.
+void EXAGEOSTAT_smle_Call(MLE_data* data, int ncores, int gpus, int dts, int p_grid, int q_grid, int N, int nZobs,
+                          int nZmiss) {
+  EXAGEOSTAT_dmle_Call(data, ncores, gpus, dts, p_grid, q_grid, N, nZobs, nZmiss);
+}
+
+void EXAGEOSTAT_sdmle_Call(MLE_data* data, int ncores, int gpus, int dts, int p_grid, int q_grid, int N, int nZobs,
+                           int nZmiss) {
+  EXAGEOSTAT_dmle_Call(data, ncores, gpus, dts, p_grid, q_grid, N, nZobs, nZmiss);
+}
+
+void EXAGEOSTAT_dmle_diag_Call(MLE_data* data, int ncores, int gpus, int dts, int p_grid, int q_grid, int N,
+                               int nZobs, int nZmiss) {
+  EXAGEOSTAT_dmle_Call(data, ncores, gpus, dts, p_grid, q_grid, N, nZobs, nZmiss);
+}
+
 void MLE_Finalize(MLE_data* data) {
   guarded([&]() {
     if (!data) return 0;
+    release_prediction_descs(data);
     free_desc(data->descC);
     free_desc(data->descZ);
     free_desc(data->descZcpy);
@@ -642,8 +925,8 @@ void EXAGEOSTAT_MLE_dzvg_Tile(MLE_data* data, double* Nrand, double* initial_the
     Context* ctx = ctx_of(data);
     const KernelId kid = kernel_from_name(data->kernel_fun);
     if (kid == kUnknownKernel) throw std::invalid_argument("unsupported kernel");
-    const long nloc = kid == kBivariate ? n / 2 : n;
-    ctx->l1.upload(data->l1.x, data->l1.y, nloc, ctx->s1);
+    const long nloc = n / kernel_vars(kid);
+    ctx->l1.upload(data->l1.x, data->l1.y, nloc, ctx->s1, data->l1.z);
     generate_covariance(ctx->C, kid, initial_theta, ctx->l1, ctx->l1, metric_of(data->dm), ctx->s1);
     const int info = tile_potrf(ctx->C, ctx->ws, ctx->s1, ctx->s2, true);
     if (info != 0) throw std::runtime_error("dzvg: covariance is not positive definite");
@@ -660,6 +943,10 @@ void EXAGEOSTAT_MLE_dzvg_Tile(MLE_data* data, double* Nrand, double* initial_the
   }, 0);
 }
 
+void EXAGEOSTAT_MLE_dzvg_Tile_Async(MLE_data* data, double* Nrand, double* initial_theta, int n, int dts, int log) {
+  EXAGEOSTAT_MLE_dzvg_Tile(data, Nrand, initial_theta, n, dts, log);
+}
+
 void MLE_zvg(MLE_data* data, double* Nrand, double* initial_theta, int n, int ts, int log, int p_grid,
              int q_grid) {
   (void)p_grid;
@@ -669,54 +956,11 @@ void MLE_zvg(MLE_data* data, double* Nrand, double* initial_theta, int n, int ts
 }
 
 double EXAGEOSTAT_dmle_Tile(unsigned n, const double* theta, double* grad, void* vdata) {
-  auto* data = static_cast<MLE_data*>(vdata);
-  const double out = guarded([&]() {
-    Context* ctx = ctx_of(data);
-    const KernelId kid = kernel_from_name(data->kernel_fun);
-    if (kid == kUnknownKernel) throw std::invalid_argument(std::string("unsupported kernel: ") + (data->kernel_fun ? data->kernel_fun : "(null)"));
-    const long nloc = kid == kBivariate ? ctx->N / 2 : ctx->N;
-    ctx->l1.upload(data->l1.x, data->l1.y, nloc, ctx->s1);
-    if (data->iter_count == 0) {
-      // Keep a pristine copy of Z for later iterations (MLE_dexact.c:303-317).
-      EXA_HIP(hipMemcpyAsync(ctx->Zcpy.d, ctx->Z.d, sizeof(double) * ctx->Z.padded, hipMemcpyDeviceToDevice,
-                             ctx->s1));
-    }
-    double logdet = 0.0, dotp = 0.0;
-    int info = 0;
-    double loglik;
-    if (grad) {
-      loglik = adjoint_loglik_grad(*ctx, kid, theta, metric_of(data->dm), grad, &logdet, &dotp, &info);
-    } else {
-      loglik = ctx->loglik(kid, theta, metric_of(data->dm), &logdet, &dotp, &info);
-    }
-    data->det = 0.5 * logdet;
-    data->dotp = dotp;
-    data->logdet = logdet;
-    data->status = info;
-    data->time_gen = ctx->last.gen;
-    data->time_facto = ctx->last.facto;
-    data->time_logdet = ctx->last.logdet;
-    data->time_solve = ctx->last.solve;
-    data->time_total = ctx->last.total;
-    if (info != 0) {
-      set_error(info, "covariance matrix is not positive definite");
-      loglik = -HUGE_VAL;
-    }
-    const int np = n ? (int)n : kernel_num_params(kid);
-    print_iteration(data, theta, np, loglik);
-    data->variance = theta[0];
-    data->iter_count++;
-    data->avg_exec_time_per_iter += ctx->last.facto + ctx->last.logdet + ctx->last.solve;
-    const double flops = (double)ctx->N * (((1.0 / 6.0) * ctx->N + 0.5) * ctx->N + 1.0 / 3.0) +
-                         (double)ctx->N * (((1.0 / 6.0) * ctx->N) * ctx->N - 1.0 / 6.0) +
-                         (double)ctx->N * (ctx->N + 1.0);
-    data->avg_flops_per_iter += flops / 1e9 / (ctx->last.facto + ctx->last.solve);
-    data->final_loglik = loglik;
-    if (info != 0) g_status = info;
-    return loglik;
-  }, -HUGE_VAL);
-  if (g_status < 0 && data) data->status = g_status;
-  return out;
+  return evaluate(static_cast<MLE_data*>(vdata), n, theta, grad, Path::kExact);
+}
+
+double EXAGEOSTAT_dmle_Tile_Async(unsigned n, const double* theta, double* grad, void* vdata) {
+  return evaluate(static_cast<MLE_data*>(vdata), n, theta, grad, Path::kExact);
 }
 
 double MORSE_dmle_Tile(unsigned n, const double* theta, double* grad, void* data) {
@@ -724,67 +968,42 @@ double MORSE_dmle_Tile(unsigned n, const double* theta, double* grad, void* data
 }
 
 double EXAGEOSTAT_sdmle_Tile(unsigned n, const double* theta, double* grad, void* vdata) {
-  auto* data = static_cast<MLE_data*>(vdata);
-  const double out = guarded([&]() {
-    Context* ctx = ctx_of(data);
-    const KernelId kid = kernel_from_name(data->kernel_fun);
-    if (kid == kUnknownKernel) throw std::invalid_argument("unsupported kernel");
-    const long nloc = kid == kBivariate ? ctx->N / 2 : ctx->N;
-    ctx->l1.upload(data->l1.x, data->l1.y, nloc, ctx->s1);
-    if (data->iter_count == 0) {
-      EXA_HIP(hipMemcpyAsync(ctx->Zcpy.d, ctx->Z.d, sizeof(double) * ctx->Z.padded, hipMemcpyDeviceToDevice,
-                             ctx->s1));
-    }
-    MixedOptions opt;
-    opt.fp64_corr = data->mp_fp64_corr;
-    opt.fp32_corr = data->mp_fp32_corr;
-    opt.bf16 = data->mp_bf16 != 0;
-    opt.ir_steps = data->mp_ir_steps;
-    opt.diag_thick = data->diag_thick;
-    opt.lolo = data->mp_lolo != 0;
-    double logdet = 0.0, dotp = 0.0;
-    int info = 0;
-    double loglik = mixed_loglik(*ctx, kid, theta, metric_of(data->dm), opt, &logdet, &dotp, &info);
-    if (grad) {
-      for (unsigned i = 0; i < n; ++i) grad[i] = std::nan("");
-    }
-    data->det = 0.5 * logdet;
-    data->dotp = dotp;
-    data->logdet = logdet;
-    data->status = info;
-    data->time_gen = ctx->last.gen;
-    data->time_facto = ctx->last.facto;
-    data->time_logdet = ctx->last.logdet;
-    data->time_solve = ctx->last.solve;
-    data->time_total = ctx->last.total;
-    if (info != 0) {
-      set_error(info, "covariance matrix is not positive definite");
-      loglik = -HUGE_VAL;
-    }
-    print_iteration(data, theta, n ? (int)n : kernel_num_params(kid), loglik);
-    data->iter_count++;
-    data->final_loglik = loglik;
-    if (info != 0) g_status = info;
-    return loglik;
-  }, -HUGE_VAL);
-  if (g_status < 0 && data) data->status = g_status;
-  return out;
+  return evaluate(static_cast<MLE_data*>(vdata), n, theta, grad, Path::kMixed);
+}
+
+double EXAGEOSTAT_sdmle_Tile_Async(unsigned n, const double* theta, double* grad, void* vdata) {
+  return evaluate(static_cast<MLE_data*>(vdata), n, theta, grad, Path::kMixed);
+}
+
+double EXAGEOSTAT_smle_Tile(unsigned n, const double* theta, double* grad, void* vdata) {
+  return evaluate(static_cast<MLE_data*>(vdata), n, theta, grad, Path::kSingle);
+}
+
+double EXAGEOSTAT_smle_Tile_Async(unsigned n, const double* theta, double* grad, void* vdata) {
+  return evaluate(static_cast<MLE_data*>(vdata), n, theta, grad, Path::kSingle);
+}
+
+double EXAGEOSTAT_dmle_diag_Tile(unsigned n, const double* theta, double* grad, void* vdata) {
+  return evaluate(static_cast<MLE_data*>(vdata), n, theta, grad, Path::kBand);
+}
+
+double EXAGEOSTAT_dmle_diag_Tile_Async(unsigned n, const double* theta, double* grad, void* vdata) {
+  return evaluate(static_cast<MLE_data*>(vdata), n, theta, grad, Path::kBand);
 }
 
 double MLE_alg(unsigned n, const double* theta, double* grad, void* vdata) {
   auto* data = static_cast<MLE_data*>(vdata);
-  if (data->computation && strcmp(data->computation, "exact") != 0) {
-    set_error(-1, "only computation=\"exact\" is provided");
-    data->status = -1;
-    return -HUGE_VAL;
-  }
-  if (data->precision == 2) return EXAGEOSTAT_sdmle_Tile(n, theta, grad, vdata);
-  if (data->precision != 0) {
-    set_error(-1, "precision must be 0 (double) or 2 (mixed)");
-    data->status = -1;
-    return -HUGE_VAL;
-  }
-  return EXAGEOSTAT_dmle_Tile(n, theta, grad, vdata);
+  const char* comp = data->computation ? data->computation : "exact";
+  const bool exact = !strcmp(comp, "exact");
+  if (exact && data->precision == 0) return EXAGEOSTAT_dmle_Tile(n, theta, grad, vdata);
+  if (exact && data->precision == 1) return EXAGEOSTAT_smle_Tile(n, theta, grad, vdata);
+  if (exact && data->precision == 2) return EXAGEOSTAT_sdmle_Tile(n, theta, grad, vdata);
+  if (!strcmp(comp, "diag_approx") && data->precision == 0) return EXAGEOSTAT_dmle_diag_Tile(n, theta, grad, vdata);
+  set_error(-1, std::string("unsupported combination: computation \"") + comp + "\" with precision " +
+                    std::to_string(data->precision) +
+                    " (exact: precision 0, 1 or 2; diag_approx: precision 0)");
+  data->status = -1;
+  return -HUGE_VAL;
 }
 
 // ------------------------------------------------------------- prediction --
@@ -800,12 +1019,23 @@ void EXAGEOSTAT_dmle_Predict_Allocate(MLE_data* data, int nZmiss, int nZobs, int
       nZobs *= 2;
       nZmiss *= 2;
     }
+    release_prediction_descs(data);
     ctx->C22.allocate(nZobs, nZobs, dts, true);
     ctx->C12.allocate(nZmiss, nZobs, dts, false);
     ctx->Zobs.allocate(nZobs, (long)ctx->C22.mt * dts);
     ctx->Zmiss.allocate(nZmiss, (long)ctx->C12.mt * dts);
     ctx->Zactual.allocate(nZmiss, (long)ctx->C12.mt * dts);
     ctx->gemv_part.allocate((long)ctx->C12.nt * ctx->C12.mt * dts, (long)ctx->C12.nt * ctx->C12.mt * dts);
+    // The descriptors of MLE_dexact.c:655-683 over this context's buffers;
+    // the MSE descriptors alias data->mserror{,1,2} as there.
+    data->descC22 = make_desc(&ctx->C22, false);
+    data->descC12 = make_desc(&ctx->C12, false);
+    data->descZobs = make_vec_desc(&ctx->Zobs, dts);
+    data->descZmiss = make_vec_desc(&ctx->Zmiss, dts);
+    data->descZactual = make_vec_desc(&ctx->Zactual, dts);
+    data->descmse = make_user_desc(&data->mserror, 1, 1);
+    data->descmse1 = make_user_desc(&data->mserror1, 1, 1);
+    data->descmse2 = make_user_desc(&data->mserror2, 1, 1);
     return 0;
   }, 0);
 }
@@ -818,6 +1048,7 @@ void prediction_finalize(MLE_data* data) {
   guarded([&]() {
     if (!data || !data->gpu) return 0;
     Context* ctx = ctx_of(data);
+    release_prediction_descs(data);
     ctx->C22.release();
     ctx->C12.release();
     ctx->Zobs.release();
@@ -839,55 +1070,75 @@ double EXAGEOSTAT_dmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, i
     Context* ctx = ctx_of(data);
     const KernelId kid = kernel_from_name(data->kernel_fun);
     if (kid == kUnknownKernel) throw std::invalid_argument("unsupported kernel");
-    if (!ctx->C22.allocated() || ctx->C22.m != nZobs || ctx->C12.m != nZmiss)
-      EXAGEOSTAT_dmle_Predict_Allocate(data, kid == kBivariate ? nZmiss / 2 : nZmiss,
-                                       kid == kBivariate ? nZobs / 2 : nZobs, ctx->ts, 1, 1, 1);
-    if (g_status < 0) throw std::runtime_error(g_error);
-    const bool biv = kid == kBivariate;
-    const long nobs_loc = biv ? nZobs / 2 : nZobs;
-    const long nmiss_loc = biv ? nZmiss / 2 : nZmiss;
+    const int vars = kernel_vars(kid);
+    predict_core(data, ctx, kid, theta, nZmiss, nZobs, data->lmiss, data->lobs, Zobs);
     hipStream_t s = ctx->s1;
-    ctx->lobs.upload(data->lobs.x, data->lobs.y, nobs_loc, s);
-    ctx->lmiss.upload(data->lmiss.x, data->lmiss.y, nmiss_loc, s);
-    const double t0 = wall_seconds();
-    generate_covariance(ctx->C22, kid, theta, ctx->lobs, ctx->lobs, metric_of(data->dm), s);
-    generate_covariance(ctx->C12, kid, theta, ctx->lmiss, ctx->lobs, metric_of(data->dm), s);
-    copy_vec_to_device(ctx->Zobs, Zobs, nZobs, s);
-    const int info = tile_potrf(ctx->C22, ctx->ws, s, ctx->s2, true);
-    if (info != 0) throw std::runtime_error("prediction: C22 is not positive definite");
-    tile_trsv(ctx->C22, ctx->ws, ctx->Zobs.d, 0, s);
-    tile_trsv(ctx->C22, ctx->ws, ctx->Zobs.d, 1, s);
-    launch_gemv_full(ctx->C12.view(), ctx->Zobs.d, ctx->Zmiss.d, ctx->gemv_part.d, s);
     EXA_HIP(hipMemcpyAsync(Zmiss, ctx->Zmiss.d, sizeof(double) * nZmiss, hipMemcpyDeviceToHost, s));
     EXA_HIP(hipStreamSynchronize(s));
-    const double t1 = wall_seconds();
-    ctx->last.total = t1 - t0;
-    data->time_total = t1 - t0;
-    data->mserror = 0.0;
-    data->mserror1 = 0.0;
-    data->mserror2 = 0.0;
     if (!Zactual) {
       data->mserror = -1;
       return -1.0;
     }
-    double se = 0.0, se1 = 0.0, se2 = 0.0;
-    for (int i = 0; i < nZmiss; ++i) {
-      const double e = (Zactual[i] - Zmiss[i]) * (Zactual[i] - Zmiss[i]);
-      se += e;
-      if (i % 2 == 0) se1 += e; else se2 += e;
-    }
-    data->mserror = se / nZmiss;
-    if (biv) {
-      data->mserror1 = se1 / (nZmiss / 2);
-      data->mserror2 = se2 / (nZmiss / 2);
-    }
-    return data->mserror;
+    return prediction_mse(data, kid, vars, Zactual, Zmiss, nZmiss);
   }, -1.0);
 }
 
 double MORSE_dmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
                                double* Zactual, double* Zmiss, int n) {
   return EXAGEOSTAT_dmle_Predict_Tile(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, n);
+}
+
+// The prediction of the single, mixed and band-limited paths is the FP64
+// predictor (MLE_sexact.c, MLE_sdexact.c and MLE_approx.c each carry a
+// prediction of their own; here one FP64 kriging serves all).
+double EXAGEOSTAT_smle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
+                                    double* Zactual, double* Zmiss, int n) {
+  return EXAGEOSTAT_dmle_Predict_Tile(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, n);
+}
+
+double EXAGEOSTAT_sdmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
+                                     double* Zactual, double* Zmiss, int n) {
+  return EXAGEOSTAT_dmle_Predict_Tile(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, n);
+}
+
+double EXAGEOSTAT_dmle_diag_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
+                                         double* Zactual, double* Zmiss, int n) {
+  return EXAGEOSTAT_dmle_Predict_Tile(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, n);
+}
+
+// MLE_dexact.c:876-1000: the held-out locations are the first nZmiss of
+// data->l1 and the observed ones the rest; Zobs and Zactual are read from
+// data->descZobs / data->descZactual (filled by the caller, e.g. with
+// EXAGEOSTAT_Lapack_to_Tile) and the predictions left in data->descZmiss.
+// Returns the MSE, or -1 when there are no actual values.
+double EXAGEOSTAT_dmle_Predict_Tile_Async(MLE_data* data, double* theta, int nZmiss, int nZobs, int n) {
+  (void)n;
+  if (nZmiss <= 0) {
+    fprintf(stderr, " Number of missing values should be positive value\n");
+    return -1;
+  }
+  return guarded([&]() {
+    Context* ctx = ctx_of(data);
+    const KernelId kid = kernel_from_name(data->kernel_fun);
+    if (kid == kUnknownKernel) throw std::invalid_argument("unsupported kernel");
+    if (!data->descZobs || !ctx->Zobs.d)
+      throw std::invalid_argument("call EXAGEOSTAT_dmle_Predict_Allocate and fill descZobs first");
+    const int vars = kernel_vars(kid);
+    const long nmiss_loc = nZmiss / vars;
+    location lmiss = data->l1;
+    location lobs{data->l1.x + nmiss_loc, data->l1.y + nmiss_loc, data->l1.z ? data->l1.z + nmiss_loc : nullptr};
+    predict_core(data, ctx, kid, theta, nZmiss, nZobs, lmiss, lobs, nullptr);
+    if (!data->descZactual) {
+      data->mserror = -1;
+      return -1.0;
+    }
+    std::vector<double> zm(nZmiss), za(nZmiss);
+    hipStream_t s = ctx->s1;
+    EXA_HIP(hipMemcpyAsync(zm.data(), ctx->Zmiss.d, sizeof(double) * nZmiss, hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipMemcpyAsync(za.data(), ctx->Zactual.d, sizeof(double) * nZmiss, hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipStreamSynchronize(s));
+    return prediction_mse(data, kid, vars, za.data(), zm.data(), nZmiss);
+  }, -1.0);
 }
 
 // ------------------------------------------------------- synthetic data --

@@ -402,9 +402,10 @@ void launch_potrf_tile(double* T, int ts, double* dinv_k, int* info, long col_ba
   EXA_HIP(hipGetLastError());
 }
 
-void launch_trsm_panel(const TileView& v, int k, const double* dinv_k, hipStream_t s) {
+void launch_trsm_panel(const TileView& v, int k, const double* dinv_k, hipStream_t s, int m_end) {
   const int q = v.ts / kNB;
-  const long slabs = (long)(v.mt - k - 1) * q;
+  if (m_end < 0 || m_end > v.mt) m_end = v.mt;
+  const long slabs = (long)(m_end - k - 1) * q;
   if (slabs <= 0) return;
   hipLaunchKernelGGL(k_trsm_panel, dim3((unsigned)slabs), dim3(256), 0, s, v, k, dinv_k);
   EXA_HIP(hipGetLastError());

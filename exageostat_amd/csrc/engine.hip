@@ -24,7 +24,15 @@ KernelId kernel_from_name(const char* name) {
   if (!strcmp(name, "univariate_matern_nuggets_stationary")) return kNugget;
   if (!strcmp(name, "bivariate_matern_parsimonious") || !strcmp(name, "bivariate_matern_parsimonious_profile"))
     return kBivariate;
+  if (!strcmp(name, "bivariate_matern_parsimonious2") || !strcmp(name, "bivariate_matern_parsimonious2_profile"))
+    return kBivariateBlock;
+  if (!strcmp(name, "univariate_spacetime_matern_stationary")) return kSpaceTime;
   return kUnknownKernel;
+}
+
+bool kernel_is_profile(const char* name) {
+  return name && (!strcmp(name, "bivariate_matern_parsimonious_profile") ||
+                  !strcmp(name, "bivariate_matern_parsimonious2_profile"));
 }
 
 int kernel_num_params(KernelId k) {
@@ -32,6 +40,8 @@ int kernel_num_params(KernelId k) {
     case kUnivariate: return 3;
     case kNugget: return 4;
     case kBivariate: return 6;
+    case kBivariateBlock: return 6;
+    case kSpaceTime: return 7;
     default: return 0;
   }
 }
@@ -94,6 +104,7 @@ void TileMatrix::set_prec(const std::vector<uint8_t>& prec) {
     for (size_t t = 0; t < prec.size(); ++t)
       if (prec_size(prec[t]) > prec_size(h_prec_[t])) throw std::invalid_argument("tile slot too small");
   }
+  if (prec == h_prec_) return;  // unchanged map: the device table is current
   h_prec_ = prec;
   // The table is read by kernels of earlier launches on the non-blocking
   // streams; the null-stream copy below does not wait for them.
@@ -213,28 +224,37 @@ static uint64_t location_fingerprint(const double* x, const double* y, long n) {
   return h;
 }
 
-void DeviceLocations::upload(const double* x, const double* y, long n_, hipStream_t s) {
-  const uint64_t f = location_fingerprint(x, y, n_);
-  if (x == hx && y == hy && n_ == n && dx && f == fp) return;
+void DeviceLocations::upload(const double* x, const double* y, long n_, hipStream_t s, const double* z) {
+  uint64_t f = location_fingerprint(x, y, n_);
+  if (z) f = (f ^ location_fingerprint(z, z, n_)) * 1099511628211ull;
+  if (x == hx && y == hy && z == hz && n_ == n && dx && f == fp) return;
   if (n_ != n || !dx) {
     release();
     n = n_;
     EXA_HIP(hipMalloc(&dx, sizeof(double) * (n > 0 ? n : 1)));
     EXA_HIP(hipMalloc(&dy, sizeof(double) * (n > 0 ? n : 1)));
   }
+  if (z && !dz) EXA_HIP(hipMalloc(&dz, sizeof(double) * (n > 0 ? n : 1)));
+  if (!z && dz) {
+    (void)hipFree(dz);
+    dz = nullptr;
+  }
   hx = x;
   hy = y;
+  hz = z;
   fp = f;
   EXA_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, s));
   EXA_HIP(hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, s));
+  if (z) EXA_HIP(hipMemcpyAsync(dz, z, sizeof(double) * n, hipMemcpyHostToDevice, s));
   EXA_HIP(hipStreamSynchronize(s));
 }
 
 void DeviceLocations::release() {
   if (dx) (void)hipFree(dx);
   if (dy) (void)hipFree(dy);
-  dx = dy = nullptr;
-  hx = hy = nullptr;
+  if (dz) (void)hipFree(dz);
+  dx = dy = dz = nullptr;
+  hx = hy = hz = nullptr;
   n = 0;
   fp = 0;
 }
@@ -244,12 +264,17 @@ void generate_covariance(TileMatrix& A, KernelId kid, const double* theta, const
                          const DeviceLocations& cols, int metric, hipStream_t s) {
   TileView v = A.view();
   const long ntiles = A.num_tiles();
+  const bool three_d = rows.dz && cols.dz && kid != kSpaceTime;
+  if (three_d && metric == 1)
+    throw std::invalid_argument("great-circle distances are 2-D only (locations carry a z coordinate)");
   if (kid == kUnivariate || kid == kNugget) {
     MaternParams P;
     P.x1 = rows.dx;
     P.y1 = rows.dy;
     P.x2 = cols.dx;
     P.y2 = cols.dy;
+    P.z1 = three_d ? rows.dz : nullptr;
+    P.z2 = three_d ? cols.dz : nullptr;
     P.metric = metric;
     P.sigma2 = theta[0];
     P.beta = theta[1];
@@ -257,13 +282,17 @@ void generate_covariance(TileMatrix& A, KernelId kid, const double* theta, const
     P.con = 1.0 / (std::pow(2.0, theta[2] - 1.0) * std::tgamma(theta[2]));
     knu_prepare(theta[2], P.ko);
     launch_gen_matern(v, P, 0, ntiles, s);
-  } else if (kid == kBivariate) {
-    if (A.ts % 2 != 0) throw std::invalid_argument("bivariate kernel needs an even tile size");
+  } else if (kid == kBivariate || kid == kBivariateBlock) {
+    if (kid == kBivariate && A.ts % 2 != 0) throw std::invalid_argument("bivariate kernel needs an even tile size");
     BivariateParams P;
     P.x1 = rows.dx;
     P.y1 = rows.dy;
     P.x2 = cols.dx;
     P.y2 = cols.dy;
+    P.z1 = three_d ? rows.dz : nullptr;
+    P.z2 = three_d ? cols.dz : nullptr;
+    P.nloc1 = rows.n;
+    P.nloc2 = cols.n;
     P.metric = metric;
     const double s1 = theta[0], s2 = theta[1], nu1 = theta[3], nu2 = theta[4], rho = theta[5];
     P.beta = theta[2];
@@ -280,7 +309,26 @@ void generate_covariance(TileMatrix& A, KernelId kid, const double* theta, const
     knu_prepare(nu1, P.ko11);
     knu_prepare(nu2, P.ko22);
     knu_prepare(nu12, P.ko12);
-    launch_gen_bivariate(v, P, 0, ntiles, s);
+    launch_gen_bivariate(v, P, 0, ntiles, kid == kBivariateBlock, s);
+  } else if (kid == kSpaceTime) {
+    if (!rows.dz || !cols.dz) throw std::invalid_argument("space-time kernel: locations need z (time)");
+    SpaceTimeParams P;
+    P.x1 = rows.dx;
+    P.y1 = rows.dy;
+    P.t1 = rows.dz;
+    P.x2 = cols.dx;
+    P.y2 = cols.dy;
+    P.t2 = cols.dz;
+    P.metric = metric;
+    P.sigma2 = theta[0];
+    P.beta_s = theta[1];
+    P.a = theta[3];
+    P.alpha = theta[4];
+    P.beta = theta[5];
+    P.delta = theta[6];
+    P.con = 1.0 / (std::pow(2.0, theta[2] - 1.0) * std::tgamma(theta[2]));
+    knu_prepare(theta[2], P.ko);
+    launch_gen_spacetime(v, P, 0, ntiles, s);
   } else {
     throw std::invalid_argument("unsupported kernel");
   }
@@ -357,7 +405,11 @@ ClassLists::~ClassLists() {
     if (p) (void)hipFree(p);
 }
 
-void ClassLists::build(const TileMatrix& A) {
+void ClassLists::build(const TileMatrix& A, int band) {
+  // The lists depend only on the tile grid, the precision map and the band:
+  // an evaluation whose map equals the previous one reuses them.
+  if (built_mt == A.mt && built_m == A.m && built_ts == A.ts && built_band == band && built_for == A.prec_map())
+    return;
   const int mt = A.mt;
   auto rows = [&](int t) { return (double)std::min<long>(A.ts, (long)A.m - (long)t * A.ts); };
   std::vector<int> pairs[4];
@@ -367,7 +419,7 @@ void ClassLists::build(const TileMatrix& A) {
   }
   for (int n = 0; n < mt; ++n) {
     for (int c = 0; c < 4; ++c) off[c][n] = (long)pairs[c].size();
-    for (int m = n; m < mt; ++m) {
+    for (int m = n; m < mt && (band <= 0 || m - n < band); ++m) {
       const int c = A.prec(m, n);
       pairs[c].push_back((m << 16) | n);
       const double f = (m == n) ? rows(n) * (rows(n) + 1.0) : 2.0 * rows(m) * rows(n);
@@ -376,19 +428,34 @@ void ClassLists::build(const TileMatrix& A) {
   }
   for (int c = 0; c < 4; ++c) {
     off[c][mt] = off[c][mt + 1] = (long)pairs[c].size();
-    if (d_pairs[c]) (void)hipFree(d_pairs[c]);
-    d_pairs[c] = nullptr;
-    if (!pairs[c].empty()) {
+    if (pairs[c].empty()) continue;
+    // Device lists keep their high-water size across maps.
+    if ((long)pairs[c].size() > cap[c]) {
+      if (d_pairs[c]) (void)hipFree(d_pairs[c]);
+      d_pairs[c] = nullptr;
+      cap[c] = 0;
       EXA_HIP(hipMalloc(&d_pairs[c], sizeof(int) * pairs[c].size()));
-      EXA_HIP(hipMemcpy(d_pairs[c], pairs[c].data(), sizeof(int) * pairs[c].size(), hipMemcpyHostToDevice));
+      cap[c] = (long)pairs[c].size();
     }
+    EXA_HIP(hipMemcpy(d_pairs[c], pairs[c].data(), sizeof(int) * pairs[c].size(), hipMemcpyHostToDevice));
   }
+  built_for = A.prec_map();
+  built_mt = A.mt;
+  built_m = A.m;
+  built_ts = A.ts;
+  built_band = band;
 }
 
 int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2, bool sync,
-               const ClassLists* lists) {
+               const ClassLists* lists, int band) {
   const int mt = A.mt;
   const int ts = A.ts;
+  if (band > 0 && !lists) throw std::invalid_argument("band-limited factorisation needs band lists");
+  if (band >= mt) band = 0;  // the band covers the whole matrix
+  // Panel rows of step k: m < row_end(k); updated columns of step k:
+  // n <= col_end(k).
+  auto row_end = [&](int k) { return band > 0 ? std::min(mt, k + band) : mt; };
+  auto col_end = [&](int k) { return band > 0 ? std::min(mt - 1, k + band - 1) : mt - 1; };
   ws.ensure(mt, ts);
   int* d_info = ws.info();
   EXA_HIP(hipMemsetAsync(d_info, 0, sizeof(int), s1));
@@ -398,13 +465,14 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
     const double rk = rows(k);
     profiled("potrf_diag", potrf_flops(rk), 0.0, s1,
              [&] { launch_potrf_tile(A.tile(k, k), ts, ws.dinv(k), d_info, (long)k * ts, s1); });
-    const double below = (double)A.m - (double)(k + 1) * ts;
-    if (k + 1 < mt)
-      profiled("trsm_panel", below * rk * rk, 0.0, s1, [&] { launch_trsm_panel(v, k, ws.dinv(k), s1); });
+    const double below = std::min((double)A.m, (double)row_end(k) * ts) - (double)(k + 1) * ts;
+    if (k + 1 < row_end(k))
+      profiled("trsm_panel", below * rk * rk, 0.0, s1,
+               [&] { launch_trsm_panel(v, k, ws.dinv(k), s1, row_end(k)); });
     // The BF16 update classes read the panel in split BF16 form.  The buffer
     // of step k is reused at step k + 2, whose panel (on s1) waits for the
     // column update of step k + 1, queued on s2 after the rest of step k.
-    if (lists && k + 1 < mt)
+    if (lists && lists->split[k & 1] && k + 1 < mt)
       profiled("split_panel", 0.0, 12.0 * below * A.ts, s1,
                [&] { launch_split_panel(v, k, lists->split[k & 1], s1); });
   };
@@ -449,9 +517,9 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
   if (lists) {
     for (int k = 0; k + 1 < mt; ++k) {
       EXA_HIP(hipStreamWaitEvent(s2, ev_panel, 0));
-      update(k, k + 1, k + 1, s2);
+      if (k + 1 <= col_end(k)) update(k, k + 1, k + 1, s2);
       hipEvent_t ev_col = record(s2);
-      if (k + 2 < mt) update(k, k + 2, mt - 1, s2);
+      if (k + 2 <= col_end(k)) update(k, k + 2, col_end(k), s2);
       EXA_HIP(hipStreamWaitEvent(s1, ev_col, 0));
       factor_panel(k + 1);
       ev_panel = record(s1);
@@ -608,14 +676,17 @@ void Context::ensure_partial(long n) {
 }
 
 double Context::loglik(KernelId kid, const double* theta, int metric, double* logdet_out, double* dotp_out,
-                       int* info) {
+                       int* info, int band) {
+  if (band < 0) throw std::invalid_argument("diag_thick must be positive");
+  const bool banded = band > 0 && band < C.mt;
   const double t0 = wall_seconds();
   EXA_HIP(hipMemcpyAsync(work.d, Zcpy.d, sizeof(double) * work.padded, hipMemcpyDeviceToDevice, s1));
   profiled("gen", 0.0, 4.0 * (double)N * (N + 1.0), s1,
            [&] { generate_covariance(C, kid, theta, l1, l1, metric, s1); });
   EXA_HIP(hipStreamSynchronize(s1));
   const double t1 = wall_seconds();
-  const int inf = tile_potrf(C, ws, s1, s2, true);
+  if (banded) band_lists.build(C, band);
+  const int inf = banded ? tile_potrf(C, ws, s1, s2, true, &band_lists, band) : tile_potrf(C, ws, s1, s2, true);
   const double t2 = wall_seconds();
   if (info) *info = inf;
   const double logdet = 2.0 * tile_logdet(C, d_partial, s1);

@@ -16,13 +16,20 @@ namespace exa {
 // Kernel families selected by the reference's kernel strings
 // (runtime/starpu/codelets/codelet_zcmg.c:262-310).
 enum KernelId {
-  kUnivariate = 0,   // "univariate_matern_stationary"
-  kBivariate = 3,    // "bivariate_matern_parsimonious"
-  kNugget = 4,       // "univariate_matern_nuggets_stationary"
+  kUnivariate = 0,       // "univariate_matern_stationary"
+  kBivariate = 3,        // "bivariate_matern_parsimonious" (variables interleaved by row parity)
+  kNugget = 4,           // "univariate_matern_nuggets_stationary"
+  kBivariateBlock = 5,   // "bivariate_matern_parsimonious2" (variable 1 rows, then variable 2 rows)
+  kSpaceTime = 6,        // "univariate_spacetime_matern_stationary" (location z = time)
   kUnknownKernel = -1,
 };
 KernelId kernel_from_name(const char* name);
 int kernel_num_params(KernelId k);
+// Rows of the covariance per location (2 for the bivariate kernels).
+inline int kernel_vars(KernelId k) { return (k == kBivariate || k == kBivariateBlock) ? 2 : 1; }
+// The "_profile" variants of the bivariate kernels: the same covariance,
+// the profile likelihood (MLE_dexact.c:403-421).
+bool kernel_is_profile(const char* name);
 
 // A tiled matrix in device memory.  Tiles are ts x ts, column-major; all
 // padding rows/columns are part of the tiles.
@@ -84,11 +91,13 @@ struct PhaseTimes {
 struct DeviceLocations {
   const double* hx = nullptr;
   const double* hy = nullptr;
+  const double* hz = nullptr;
   long n = 0;
   uint64_t fp = 0;
   double* dx = nullptr;
   double* dy = nullptr;
-  void upload(const double* x, const double* y, long n_, hipStream_t s);
+  double* dz = nullptr;  // third coordinate (3-D locations, or time), or null
+  void upload(const double* x, const double* y, long n_, hipStream_t s, const double* z = nullptr);
   void release();
   ~DeviceLocations() { release(); }
 };
@@ -144,15 +153,27 @@ struct ClassLists {
   uint16_t* split[2] = {nullptr, nullptr};
   bool lolo = false;  // kF32 updates with four split-BF16 products
   ~ClassLists();
-  void build(const TileMatrix& A);
+  // Rebuilds the lists unless A's grid, precision map and band are the ones
+  // they were last built for.  band > 0 keeps only the pairs m - n < band.
+  void build(const TileMatrix& A, int band = 0);
+
+ private:
+  long cap[4] = {0, 0, 0, 0};  // device list capacities (pairs)
+  std::vector<uint8_t> built_for;
+  int built_mt = -1, built_m = -1, built_ts = -1, built_band = -1;
 };
 
 // Right-looking tile Cholesky with one-step look-ahead on two streams.
 // Returns 0 or the first failing global column + 1 (the matrix is then
 // not positive definite).  Synchronises s_main at the end.
 // With lists, the trailing updates run per precision class (mixed factor).
+// band > 0 (with lists built for that band): the band-limited
+// factorisation of CHAM_pdpotrf_diag (exageostat_approx/src/compute/
+// pdpotrf_diag.c:64-110): step k solves only the panel tiles m < k + band
+// and updates only the columns n < k + band, tiles m < n + band; tiles
+// outside the band keep the covariance.
 int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s_main, hipStream_t s_aux, bool sync = true,
-               const ClassLists* lists = nullptr);
+               const ClassLists* lists = nullptr, int band = 0);
 // z <- L^-1 z (trans=0) or L^-T z (trans=1) with the factor from tile_potrf.
 void tile_trsv(const TileMatrix& L, const CholWorkspace& ws, double* z, int trans, hipStream_t s);
 // sum_i log L_ii (deterministic order).
@@ -187,8 +208,13 @@ class Context {
   MixedState* mixed = nullptr;  // mixed-precision factor, created on first use
   AdjointState* adjoint = nullptr;  // gradient buffers, created on first use
   MloeState* mloe = nullptr;        // MLOE / MMOM buffers, created on first use
+  ClassLists band_lists;            // update pairs of the band-limited factorisation
 
-  double loglik(KernelId kid, const double* theta, int metric, double* logdet_out, double* dotp_out, int* info);
+  // band > 0: the band-limited factorisation (computation "diag_approx");
+  // the log-determinant and the solve then read the whole lower triangle,
+  // whose tiles outside the band still hold the covariance.
+  double loglik(KernelId kid, const double* theta, int metric, double* logdet_out, double* dotp_out, int* info,
+                int band = 0);
   void ensure_partial(long n);
 };
 

@@ -32,6 +32,14 @@ __device__ inline void store_prec(void* base, long idx, double v, uint8_t prec) 
   }
 }
 
+// Euclidean distance in 3-D when both location sets carry a third
+// coordinate (calculateDistance, core_zcmg.c:71-92); great-circle
+// distances are 2-D only.
+__device__ inline double pair_distance3(double x1, double y1, double z1, double x2, double y2, double z2) {
+  const double dx = x2 - x1, dy = y2 - y1, dz = z2 - z1;
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
 __device__ inline double pair_distance(double x1, double y1, double x2, double y2, int metric) {
   if (metric == 1) {
     // Haversine great-circle distance in km (core_zcmg.c:51-60).
@@ -74,10 +82,11 @@ __global__ void __launch_bounds__(256) k_gen_matern(TileView v, MaternParams P, 
   const int ts = v.ts;
   const int i = bi * kNB + threadIdx.x;  // row inside the tile
   const long gi = (long)m * ts + i;
-  double xi = 0.0, yi = 0.0;
+  double xi = 0.0, yi = 0.0, zi = 0.0;
   if (gi < v.m) {
     xi = P.x1[gi];
     yi = P.y1[gi];
+    if (P.z1) zi = P.z1[gi];
   }
   for (int cc = threadIdx.y; cc < kNB; cc += 4) {
     const int j = bj * kNB + cc;
@@ -86,7 +95,8 @@ __global__ void __launch_bounds__(256) k_gen_matern(TileView v, MaternParams P, 
     if (gi >= v.m || gj >= v.n) {
       val = (gi == gj) ? 1.0 : 0.0;
     } else {
-      const double d = pair_distance(xi, yi, P.x2[gj], P.y2[gj], P.metric);
+      const double d = P.z1 ? pair_distance3(xi, yi, zi, P.x2[gj], P.y2[gj], P.z2[gj])
+                            : pair_distance(xi, yi, P.x2[gj], P.y2[gj], P.metric);
       const double x = d / P.beta;
       if (x == 0.0) {
         val = P.sigma2 + P.nugget;
@@ -114,10 +124,11 @@ __global__ void __launch_bounds__(256) k_gen_bivariate(TileView v, BivariatePara
   const int rp = threadIdx.x & 31;
   const int i = bi * kNB + 2 * rp;
   const long gi = (long)m * ts + i;
-  double xi = 0.0, yi = 0.0;
+  double xi = 0.0, yi = 0.0, zi = 0.0;
   if (gi < v.m) {
     xi = P.x1[gi >> 1];
     yi = P.y1[gi >> 1];
+    if (P.z1) zi = P.z1[gi >> 1];
   }
   for (int cp = (threadIdx.x >> 5) + 2 * threadIdx.y; cp < kNB / 2; cp += 8) {
     const int j = bj * kNB + 2 * cp;
@@ -129,7 +140,9 @@ __global__ void __launch_bounds__(256) k_gen_bivariate(TileView v, BivariatePara
       a10 = (gi + 1 == gj) ? 1.0 : 0.0;
       a01 = (gi == gj + 1) ? 1.0 : 0.0;
     } else {
-      const double d = pair_distance(xi, yi, P.x2[gj >> 1], P.y2[gj >> 1], P.metric);
+      const long q = gj >> 1;
+      const double d = P.z1 ? pair_distance3(xi, yi, zi, P.x2[q], P.y2[q], P.z2[q])
+                            : pair_distance(xi, yi, P.x2[q], P.y2[q], P.metric);
       const double x = d / P.beta;
       if (x == 0.0) {
         a00 = P.sigma11;
@@ -148,6 +161,106 @@ __global__ void __launch_bounds__(256) k_gen_bivariate(TileView v, BivariatePara
   }
 }
 
+// Parsimonious bivariate Matern in the block layout of the reference's
+// "bivariate_matern_parsimonious2" (three univariate generations into the
+// quadrants of C, MLE_dexact.c:323-360): row r < nloc1 is variable 1 at
+// location r, row nloc1 + r variable 2 at location r (columns likewise with
+// nloc2).  One thread per entry, 64 consecutive rows per column.
+__global__ void __launch_bounds__(256) k_gen_bivariate_block(TileView v, BivariateParams P, int first_col) {
+  int m, n, bi, bj;
+  decode_gen_block(v, first_col, m, n, bi, bj);
+  const long t = tile_index(v, m, n);
+  void* base = v.ptr[t];
+  const uint8_t prec = v.prec ? v.prec[t] : (uint8_t)kF64;
+  const int ts = v.ts;
+  const int i = bi * kNB + threadIdx.x;
+  const long gi = (long)m * ts + i;
+  int va = 0;
+  double xi = 0.0, yi = 0.0, zi = 0.0;
+  if (gi < v.m) {
+    va = gi >= P.nloc1 ? 1 : 0;
+    const long p = gi - va * P.nloc1;
+    xi = P.x1[p];
+    yi = P.y1[p];
+    if (P.z1) zi = P.z1[p];
+  }
+  for (int cc = threadIdx.y; cc < kNB; cc += 4) {
+    const int j = bj * kNB + cc;
+    const long gj = (long)n * ts + j;
+    double val;
+    if (gi >= v.m || gj >= v.n) {
+      val = (gi == gj) ? 1.0 : 0.0;
+    } else {
+      const int vb = gj >= P.nloc2 ? 1 : 0;
+      const long q = gj - vb * P.nloc2;
+      const double d = P.z1 ? pair_distance3(xi, yi, zi, P.x2[q], P.y2[q], P.z2[q])
+                            : pair_distance(xi, yi, P.x2[q], P.y2[q], P.metric);
+      const double x = d / P.beta;
+      const int c = va + vb;  // 0: var 1 with var 1, 1: cross, 2: var 2 with var 2
+      const double sg = c == 0 ? P.sigma11 : (c == 1 ? P.sigma12 : P.sigma22);
+      if (x == 0.0) {
+        val = sg;
+      } else if (c == 0) {
+        val = sg * matern_corr(P.ko11, P.con11, x);
+      } else if (c == 1) {
+        val = sg * matern_corr(P.ko12, P.con12, x);
+      } else {
+        val = sg * matern_corr(P.ko22, P.con22, x);
+      }
+    }
+    store_prec(base, (long)i + (long)j * ts, val, prec);
+  }
+}
+
+// Space-time Matern (core_zcmg.c:1320-1357): with the spatial distance d
+// (2-D; the time coordinate is not part of it) and u = |t_i - t_j|,
+//   psi = u^(2 alpha) / a + 1,  x = d / (beta_s psi^(beta/2)),
+//   C = sigma^2 M_nu(x) / psi^(beta + delta),   C = sigma^2 / psi^(beta + delta) at d = 0.
+__global__ void __launch_bounds__(256) k_gen_spacetime(TileView v, SpaceTimeParams P, int first_col) {
+  int m, n, bi, bj;
+  decode_gen_block(v, first_col, m, n, bi, bj);
+  const long t = tile_index(v, m, n);
+  void* base = v.ptr[t];
+  const uint8_t prec = v.prec ? v.prec[t] : (uint8_t)kF64;
+  const int ts = v.ts;
+  const int i = bi * kNB + threadIdx.x;
+  const long gi = (long)m * ts + i;
+  double xi = 0.0, yi = 0.0, ti = 0.0;
+  if (gi < v.m) {
+    xi = P.x1[gi];
+    yi = P.y1[gi];
+    ti = P.t1[gi];
+  }
+  for (int cc = threadIdx.y; cc < kNB; cc += 4) {
+    const int j = bj * kNB + cc;
+    const long gj = (long)n * ts + j;
+    double val;
+    if (gi >= v.m || gj >= v.n) {
+      val = (gi == gj) ? 1.0 : 0.0;
+    } else {
+      const double xs = pair_distance(xi, yi, P.x2[gj], P.y2[gj], P.metric) / P.beta_s;
+      const double u = fabs(ti - P.t2[gj]);
+      const double psi = pow(u, 2.0 * P.alpha) / P.a + 1.0;
+      const double den = pow(psi, P.beta + P.delta);
+      if (xs == 0.0) {
+        val = P.sigma2 / den;
+      } else {
+        const double x = xs / pow(psi, 0.5 * P.beta);
+        val = P.sigma2 * matern_corr(P.ko, P.con, x) / den;
+      }
+    }
+    store_prec(base, (long)i + (long)j * ts, val, prec);
+  }
+}
+
+void launch_gen_spacetime(const TileView& v, const SpaceTimeParams& P, int first_col, long ntiles, hipStream_t s) {
+  const int q = v.ts / kNB;
+  const long blocks = ntiles * q * q;
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(k_gen_spacetime, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, first_col);
+  EXA_HIP(hipGetLastError());
+}
+
 void launch_gen_matern(const TileView& v, const MaternParams& P, int first_col, long ntiles, hipStream_t s) {
   const int q = v.ts / kNB;
   const long blocks = ntiles * q * q;
@@ -156,11 +269,15 @@ void launch_gen_matern(const TileView& v, const MaternParams& P, int first_col, 
   EXA_HIP(hipGetLastError());
 }
 
-void launch_gen_bivariate(const TileView& v, const BivariateParams& P, int first_col, long ntiles, hipStream_t s) {
+void launch_gen_bivariate(const TileView& v, const BivariateParams& P, int first_col, long ntiles, bool block,
+                          hipStream_t s) {
   const int q = v.ts / kNB;
   const long blocks = ntiles * q * q;
   if (blocks <= 0) return;
-  hipLaunchKernelGGL(k_gen_bivariate, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, first_col);
+  if (block)
+    hipLaunchKernelGGL(k_gen_bivariate_block, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, first_col);
+  else
+    hipLaunchKernelGGL(k_gen_bivariate, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, first_col);
   EXA_HIP(hipGetLastError());
 }
 

@@ -13,6 +13,8 @@ struct MaternParams {
   const double* y1;
   const double* x2;
   const double* y2;
+  const double* z1;  // third coordinates (3-D Euclidean distance), or null
+  const double* z2;
   int metric;     // 0 Euclidean, 1 great circle
   double beta;    // range
   double sigma2;  // variance
@@ -26,6 +28,9 @@ struct BivariateParams {
   const double* y1;
   const double* x2;
   const double* y2;
+  const double* z1;  // third coordinates, or null
+  const double* z2;
+  long nloc1, nloc2;  // locations of the row / column sets (block layout)
   int metric;
   double beta;
   double sigma11, sigma22, sigma12;  // variances and rho~ sqrt(s1 s2)
@@ -33,10 +38,31 @@ struct BivariateParams {
   KnuOrder<double> ko11, ko22, ko12;
 };
 
+// Space-time Matern of Gneiting (2002, eq. 16) as core_dcmg_spacetime_matern
+// writes it (exageostat_exact/cpu_core/compute/core_zcmg.c:1320-1357):
+// location z is time, theta = (sigma^2, spatial range, nu, time range a,
+// time smoothness alpha, non-separability beta, delta).
+struct SpaceTimeParams {
+  const double* x1;
+  const double* y1;
+  const double* t1;
+  const double* x2;
+  const double* y2;
+  const double* t2;
+  int metric;
+  double sigma2, beta_s, a, alpha, beta, delta;
+  double con;  // 1/(2^(nu-1) Gamma(nu))
+  KnuOrder<double> ko;
+};
+
 // Generation: ntiles tiles starting at tile column first_col (lower
 // matrices: the trailing triangle from first_col; full matrices: all).
 void launch_gen_matern(const TileView& v, const MaternParams& P, int first_col, long ntiles, hipStream_t s);
-void launch_gen_bivariate(const TileView& v, const BivariateParams& P, int first_col, long ntiles, hipStream_t s);
+// block = false: variables interleaved by row parity; true: the rows of
+// variable 1 (P.nloc1 of them), then those of variable 2.
+void launch_gen_bivariate(const TileView& v, const BivariateParams& P, int first_col, long ntiles, bool block,
+                          hipStream_t s);
+void launch_gen_spacetime(const TileView& v, const SpaceTimeParams& P, int first_col, long ntiles, hipStream_t s);
 
 // ---- tile Cholesky building blocks (FP64) -------------------------------
 // Factor the diagonal tile (k,k) of a lower-packed matrix in place with
@@ -44,8 +70,8 @@ void launch_gen_bivariate(const TileView& v, const BivariateParams& P, int first
 // L_kk to dinv_k (q blocks of 64x64, column-major) and the first failing
 // global column + 1 to *info (left untouched on success).
 void launch_potrf_tile(double* T, int ts, double* dinv_k, int* info, long col_base, hipStream_t s);
-// A(m,k) <- A(m,k) L_kk^-T for all m in (k, mt).
-void launch_trsm_panel(const TileView& v, int k, const double* dinv_k, hipStream_t s);
+// A(m,k) <- A(m,k) L_kk^-T for all m in (k, m_end) (m_end < 0: mt).
+void launch_trsm_panel(const TileView& v, int k, const double* dinv_k, hipStream_t s, int m_end = -1);
 // A(m,n) -= sum_{p < np} A(m,k+p) A(n,k+p)^T for n in [col_lo, col_hi],
 // m in [n, mt); np = 1 or 2 (col_lo >= k + np).
 void launch_tile_update(const TileView& v, int k, int np, int col_lo, int col_hi, hipStream_t s);

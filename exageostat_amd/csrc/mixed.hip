@@ -146,8 +146,9 @@ static void symv(const TileMatrix& C, const double* x, double* y, double* part, 
 }
 
 double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric, const MixedOptions& opt,
-                    double* logdet_out, double* dotp_out, int* info) {
+                    double* logdet_out, double* dotp_out, int* info, MixedReport* rep) {
   if (opt.ir_steps < 0) throw std::invalid_argument("ir_steps must be >= 0");
+  if (!(opt.ir_tol >= 0.0)) throw std::invalid_argument("ir_tol must be >= 0");
   if (!ctx.mixed) ctx.mixed = new MixedState();
   MixedState& st = *ctx.mixed;
   const int N = ctx.N, ts = ctx.ts, mt = ctx.C.mt;
@@ -199,6 +200,7 @@ double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric,
              [&] { tile_trsv(st.Cm, ctx.ws, v, 1, s1); });
   };
   EXA_HIP(hipMemcpyAsync(x, z, sizeof(double) * padded, hipMemcpyDeviceToDevice, s1));
+  MixedReport report;
   if (inf == 0) {
     solve(x);
     for (int it = 0; it < opt.ir_steps; ++it) {
@@ -207,8 +209,19 @@ double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric,
       launch_axpby(1.0, z, -1.0, r, r, N, s1);
       solve(r);
       launch_axpby(1.0, x, 1.0, r, x, N, s1);
+      ++report.ir_done;
+      if (opt.ir_tol > 0.0) {
+        // The step changed z^T x by z^T c (c = r, the correction just
+        // added); stop once that is below the tolerance.
+        const double dq = device_dot(z, r, N, ctx.d_partial, s1);
+        const double q = device_dot(z, x, N, ctx.d_partial, s1);
+        report.ir_change = q != 0.0 ? std::fabs(dq) / std::fabs(q) : std::fabs(dq);
+        if (report.ir_change <= opt.ir_tol) break;
+      }
     }
+    report.ir_converged = !(opt.ir_tol > 0.0 && opt.ir_steps > 0 && report.ir_change > opt.ir_tol);
   }
+  if (rep) *rep = report;
   const double dotp = device_dot(z, x, N, ctx.d_partial, s1);
   // ctx.work receives Sigma^-1 z (the FP64 path leaves L^-1 z there).
   EXA_HIP(hipMemcpyAsync(ctx.work.d, x, sizeof(double) * padded, hipMemcpyDeviceToDevice, s1));

@@ -10,13 +10,21 @@ struct MixedOptions {
   double fp64_corr = 0.0;  // off-diagonal tiles with max correlation >= this stay FP64 (0: none)
   double fp32_corr = 0.0;  // ... >= this are FP32, the rest BF16 (when bf16)
   bool bf16 = false;
-  int ir_steps = 3;
+  int ir_steps = 3;   // at most this many FP64 refinement steps
+  double ir_tol = 0;  // > 0: stop once a step changes z^T x by <= ir_tol (relative)
   int diag_thick = 1;
   bool lolo = false;  // FP32 class: hi*hi + hi*lo + lo*hi (+ lo*lo when set)
 };
 
+// What the refinement did in the last mixed evaluation.
+struct MixedReport {
+  int ir_done = 0;         // refinement steps run
+  double ir_change = 0.0;  // |z^T c| / |z^T x| of the last step's correction c (0: no step)
+  bool ir_converged = true;  // false: ir_tol > 0 and the last change is above it
+};
+
 double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric, const MixedOptions& opt,
-                    double* logdet_out, double* dotp_out, int* info);
+                    double* logdet_out, double* dotp_out, int* info, MixedReport* rep = nullptr);
 // Copies the tile precision map (TilePrec, lower-packed tile order) of the
 // last mixed evaluation into out[0:cap); returns the tile count (0: none).
 long mixed_prec_map(const Context& ctx, uint8_t* out, long cap);

@@ -148,16 +148,21 @@ class ExactMLE:
         th = np.ascontiguousarray(theta, dtype=np.float64)
         g = np.zeros(len(th)) if grad else None
         val = self.lib.MLE_alg(len(th), _dp(th), _dp(g) if grad else None, ctypes.byref(self.data))
-        _check(self.lib, "MLE_alg")
+        if self.lib.exageostat_gpu_last_status() != -3:
+            # -3: the mixed refinement did not reach mp_ir_tol; l is still
+            # returned and self.status says so
+            _check(self.lib, "MLE_alg")
         return (val, g) if grad else val
 
     def set_mixed(self, fp32_corr: float = 0.0, bf16: bool = False, fp64_corr: float = 0.0,
-                  diag_thick: int = 1, ir_steps: int = 3, lolo: bool = False) -> None:
+                  diag_thick: int = 1, ir_steps: int = 3, lolo: bool = False, ir_tol: float = 0.0) -> None:
         """Mixed-precision options (used when precision == 2): tiles whose
         largest correlation is >= fp64_corr stay FP64, tiles below fp32_corr
         get BF16 updates when bf16 is on, the rest are FP32; the solve is
-        refined ir_steps times in FP64.  lolo: the FP32 tiles' split-BF16
-        products keep their lo*lo term (four products instead of three)."""
+        refined at most ir_steps times in FP64, stopping early once a step
+        changes z^T Sigma^-1 z by at most ir_tol (relative; 0: always
+        ir_steps steps).  lolo: the FP32 tiles' split-BF16 products keep
+        their lo*lo term (four products instead of three)."""
         d = self.data
         d.mp_fp32_corr = fp32_corr
         d.mp_bf16 = 1 if bf16 else 0
@@ -165,6 +170,7 @@ class ExactMLE:
         d.diag_thick = diag_thick
         d.mp_ir_steps = ir_steps
         d.mp_lolo = 1 if lolo else 0
+        d.mp_ir_tol = ir_tol
 
     @property
     def precision(self) -> int:

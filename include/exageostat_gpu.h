@@ -17,12 +17,14 @@
  *  - descriptors are this library's own device tile matrices: the struct is
  *    source-compatible with the fields the hot path reads (m, n, mb, nb, mt,
  *    nt, mat, dtyp), not binary-compatible with Chameleon's CHAM_desc_t;
- *  - MLE_data is the subset of the reference struct
- *    (misc/include/MLE_misc.h:184-328) that the path reads or writes, plus
- *    a few fields of this implementation at the end.
+ *  - MLE_data has every field of the reference struct, in its order
+ *    (misc/include/MLE_misc.h:184-328), plus fields of this implementation
+ *    at the end.
  */
 #ifndef EXAGEOSTAT_GPU_H
 #define EXAGEOSTAT_GPU_H
+
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -70,59 +72,212 @@ typedef struct exageostat_request_s {
   int status;
 } RUNTIME_request_t;
 
-/* Subset of MLE_data (misc/include/MLE_misc.h:184-328); field names and
- * meanings as in the reference. */
+/* misc/include/MLE_misc.h:130-178: the summary a driver fills and prints.
+ * The library writes final_loglik and estimated_theta (when the caller has
+ * allocated it) after every likelihood evaluation, and the prediction, MLOE
+ * and Fisher fields after those calls, as the reference does. */
+typedef struct {
+  int problem_size;
+  char *computation;
+  char *kernel;
+  char *ds_type;
+  char *precision;
+  int z_sample;
+  int dense_ts;
+  int lr_ts;
+  int lr_acc;
+  int lr_maxrank;
+  int ncores;
+  int ngpus;
+  int p;
+  int q;
+  int num_params;
+  double *initial_theta;
+  double *starting_theta;
+  double *estimated_theta;
+  double final_loglik;
+  double time_per_iteration;
+  double cholesky_per_iter;
+  double flops_per_iteration;
+  double generation_per_iter;
+  double solve_per_iter;
+  double total_mle_time;
+  double avg_exec_time_gen_stage;
+  double avg_flops_gen_stage;
+  double mse_pred1;
+  double mse_pred2;
+  double mse_pred3;
+  double mse_pred;
+  double trace_pred_sum;
+  double trace_pred_mean;
+  double total_pred_time;
+  double total_pred_flops;
+  double mloe;
+  double mmom;
+  char *mloe_exec;
+  double total_mloe_mmom_time;
+  double matrix_gen_mloe_mmom_time;
+  double cho_fact_mloe_mmom_time;
+  double loop_mloe_mmom_time;
+  double total_mloe_mmom_flops;
+  double fisher_00;
+  double fisher_11;
+  double fisher_22;
+  double fisher_time;
+} output;
+
+/* misc/include/MLE_misc.h:350 (defined by the library; a driver that defines
+ * its own `output results;`, as examples/examples.c:22 does, takes
+ * precedence). */
+extern output results;
+
+/* MLE_data: every field of the reference struct in the reference's order
+ * (misc/include/MLE_misc.h:184-328), so the reference's drivers and their
+ * argument parsing (examples/examples.c) compile unchanged; fields of this
+ * implementation follow at the end.  Fields the exact-dense GPU path does
+ * not use (HiCMA, out-of-core, non-Gaussian, file paths of real datasets
+ * other than through readLocsFile/readObsFile) are accepted and ignored. */
 typedef struct {
   double variance;
   double variance1;
   double variance2;
-  char *computation;    /* "exact"                                          */
+  double variance3;
+  char *computation;    /* "exact" or "diag_approx"                          */
+  char *c_fun;
   int test;
-  int async;
+  int async;            /* 1: the *_Async entry points (same results here)  */
   int iter_count;
   location l1;          /* locations of the observations                    */
   location lmiss;       /* prediction: held-out locations                   */
   location lobs;        /* prediction: observed locations                   */
   location lm;
   void *descC;          /* Sigma, lower tiles                               */
+  void *descsubC11;
+  void *descsubC12;
+  void *descsubC21;
+  void *descsubC22;
   void *descZ;
+  void *descZ1;
+  void *descZ2;
+  void *descZ3;
+  double *Adense;
+  double *Adense2;
   void *descZcpy;
   void *descdet;
+  void *descsum;
   void *descproduct;
+  void *descproduct1;
+  void *descproduct2;
+  void *descproduct3;
   void *descZmiss;
+  void *descC11;
+  void *descC21;
   void *descC12;
   void *descC22;
   void *descZactual;
   void *descZobs;
   void *descmse;
+  void *descmse1;
+  void *descmse2;
+  void *descmse3;
+  void *descZtrace;
   void *sequence;
   void *request;
   int verbose;
-  int log;
+  int check;
+  int log;              /* 1: iteration lines also go to pFileLog            */
   double avg_exec_time_per_iter;
   double total_exec_time;
   double avg_flops_per_iter;
   double avg_generation_per_iter;
   double avg_solve_per_iter;
   double avg_cholesky_per_iter;
+  double avg_exec_time_gen_stage;
+  double avg_flops_gen_stage;
   double final_loglik;
-  char *recovery_file;
-  char *checkpoint_file;
+  char *locsFPath;
+  char *timeFPath;
+  char *obsFPath;
+  char *obsFPath2;
+  char *obsFPath3;
+  char *actualZFPath;
+  char *actualZFPath2;
+  char *actualZFPath3;
+  char *actualZLocFPath;
+  char *actualTimeFPath;
   double det;           /* sum of log L_ii (logdet = 2*det)                  */
+  double sum;
   double dotp;          /* ||L^-1 z||^2                                      */
+  double dotp1;
+  double dotp2;
+  double dotp3;
+  float sdotp;          /* precision 1: the dot product in FP32              */
   double mserror;
   double mserror1;
   double mserror2;
+  double mserror3;
   char *dm;             /* "ed" Euclidean, "gc" great circle                 */
-  int diag_thick;       /* mixed precision: tile band kept in FP64           */
+  int diag_thick;       /* diag_approx: tile band factored; mixed: FP64 band */
+  char *nFileLog;
+  FILE *pFileLog;
+  int hicma_maxrank;
+  int hicma_data_type;
+  void *hicma_descC;
+  void *hicma_descZ;
+  void *hicma_descCD;
+  void *hicma_descCUV;
+  void *hicma_descCrk;
+  void *hicma_descZcpy;
+  void *hicma_descdet;
+  void *hicma_descproduct;
+  void *hicma_descC12D;
+  void *hicma_descC12UV;
+  void *hicma_descC12rk;
+  void *hicma_descC22D;
+  void *hicma_descC22UV;
+  void *hicma_descC22rk;
+  double hicma_acc;
+  void *hsequence;
+  void *hrequest;
   int opt_tol;
   int opt_max_iters;
+  int ooc;
   char *kernel_fun;     /* "univariate_matern_stationary", ...              */
-  int precision;        /* 0 double, 2 mixed (FP64 / FP32 / BF16 tiles)      */
-  int mspe;
-  int seed;
+  int precision;        /* 0 double, 1 single, 2 mixed (FP64/FP32/BF16 tiles) */
+  void *desctemp;
+  void *desctemp22;
+  void *desck_t;
+  void *desck_a;
+  void *desck_ttmp;
+  void *desck_atmp;
+  void *descK_t;
+  void *descK_a;
+  void *descexpr1;
+  void *descexpr2;
+  void *descexpr3;
+  void *descexpr4;
+  void *descestimatedalpha;
+  void *desctruthalpha;
+  void *desc_mloe_mmom;
+  void *descmloe;
+  void *descmmom;
+  double expr1;
+  double expr2;
+  double expr3;
+  double expr4;
   double mloe;          /* MLOE of the last EXAGEOSTAT_dmle_mloe_mmom_Tile     */
   double mmom;          /* MMOM of the last EXAGEOSTAT_dmle_mloe_mmom_Tile     */
+  int mloe_mmom;
+  int mloe_mmom_async;
+  int mspe;
+  char *recovery_file;  /* "": none; else replayed / appended per evaluation */
+  char *checkpoint_file;
+  int time_slots;
+  int idw;
+  int fisher;
+  void *descr;
+  void *descrcpy;
+  int seed;
   /* ---- fields of this implementation ---- */
   void *gpu;            /* engine context (EXAGEOSTAT_dmle_Call / MLE_Finalize) */
   int silent;           /* 1: no per-iteration line on stderr                */
@@ -136,12 +291,18 @@ typedef struct {
   double mp_fp32_corr;  /* mixed: off-diagonal tiles whose largest
                            correlation is >= this stay FP32, others BF16     */
   int mp_bf16;          /* mixed: 1 enables the BF16 tier                    */
-  int mp_ir_steps;      /* mixed: FP64 iterative-refinement steps on the solve */
+  int mp_ir_steps;      /* mixed: at most this many FP64 refinement steps    */
   double mp_fp64_corr;  /* mixed: off-diagonal tiles whose largest
                            correlation is >= this stay FP64                  */
   int mp_lolo;          /* mixed: 1 adds the lo*lo term to the split-BF16
                            products of the FP32 tiles (four products instead
                            of three: smaller log-det bias, ~6% slower)       */
+  double mp_ir_tol;     /* mixed: > 0 stops the refinement once a step
+                           changes z^T Sigma^-1 z by <= mp_ir_tol (relative);
+                           if mp_ir_steps steps do not get there, status is
+                           -3 (the likelihood is still returned)             */
+  int mp_ir_done;       /* mixed, output: refinement steps run               */
+  double mp_ir_change;  /* mixed, output: relative change of the last step   */
 } MLE_data;
 
 /* ---------------------------------------------------------------- setup -- */
@@ -152,6 +313,16 @@ void exageostat_finalize(void);
 /* exageostat_exact/src/compute/MLE_dexact.c:1938-2017. */
 void EXAGEOSTAT_dmle_Call(MLE_data *data, int ncores, int gpus, int dts, int p_grid, int q_grid, int N,
                           int nZobs, int nZmiss);
+/* exageostat_exact/src/include/MLE_exact_s.h:59-61, MLE_sdexact.h:63-65,
+ * exageostat_approx/src/include/MLE_approx.h:54-56: the same device context
+ * as EXAGEOSTAT_dmle_Call (the precision of each tile is chosen per
+ * evaluation). */
+void EXAGEOSTAT_smle_Call(MLE_data *data, int ncores, int gpus, int dts, int p_grid, int q_grid, int N, int nZobs,
+                          int nZmiss);
+void EXAGEOSTAT_sdmle_Call(MLE_data *data, int ncores, int gpus, int dts, int p_grid, int q_grid, int N, int nZobs,
+                           int nZmiss);
+void EXAGEOSTAT_dmle_diag_Call(MLE_data *data, int ncores, int gpus, int dts, int p_grid, int q_grid, int N,
+                               int nZobs, int nZmiss);
 /* src/compute/MLE.c:381-401. */
 void MLE_Finalize(MLE_data *data);
 /* src/compute/MLE.c:420-434. */
@@ -167,18 +338,50 @@ void pick_random_points(MLE_data *data, double *Zobs, double *Zactual, int nZmis
 void MLE_zvg(MLE_data *data, double *Nrand, double *initial_theta, int n, int ts, int log, int p_grid,
              int q_grid);
 void EXAGEOSTAT_MLE_dzvg_Tile(MLE_data *data, double *Nrand, double *initial_theta, int n, int dts, int log);
+/* MLE_exact.h:38-39 (MLE_dexact.c:167-229); stream-ordered, same result. */
+void EXAGEOSTAT_MLE_dzvg_Tile_Async(MLE_data *data, double *Nrand, double *initial_theta, int n, int dts, int log);
 
 /* ------------------------------------------------------------ likelihood -- */
 /* src/include/MLE.h:48-49, src/compute/MLE.c:138-193: NLOPT objective.
- * grad, when non-NULL, receives d l / d theta (the reference ignores it). */
+ * Dispatches as the reference does: computation "exact" with precision 0
+ * (EXAGEOSTAT_dmle_Tile), 1 (EXAGEOSTAT_smle_Tile) or 2
+ * (EXAGEOSTAT_sdmle_Tile); "diag_approx" with precision 0
+ * (EXAGEOSTAT_dmle_diag_Tile).  Other combinations return -HUGE_VAL with
+ * status -1 (the reference returns 0).  grad, when non-NULL, receives
+ * d l / d theta on the precision-0 exact path (the reference ignores it);
+ * the other paths then fail with status -1.  The "_profile" bivariate
+ * kernels return the profile likelihood and set variance1 / variance2
+ * (MLE_dexact.c:403-421).  With a non-empty recovery_file, an evaluation
+ * whose iteration number has a line there returns that line's likelihood
+ * without computing (MLE_dexact.c:306-307); with a non-empty
+ * checkpoint_file every computed evaluation appends its line. */
 double MLE_alg(unsigned n, const double *theta, double *grad, void *data);
-/* exageostat_exact/src/include/MLE_exact.h:43-44, MLE_dexact.c:231-489. */
+/* exageostat_exact/src/include/MLE_exact.h:43-47, MLE_dexact.c:231-611.
+ * The _Async variant is stream-ordered here and returns the same value. */
 double EXAGEOSTAT_dmle_Tile(unsigned n, const double *theta, double *grad, void *data);
+double EXAGEOSTAT_dmle_Tile_Async(unsigned n, const double *theta, double *grad, void *data);
 double MORSE_dmle_Tile(unsigned n, const double *theta, double *grad, void *data);
-/* exageostat_exact/src/include/MLE_sdexact.h (EXAGEOSTAT_sdmle_Tile,
+/* exageostat_exact/src/include/MLE_sdexact.h:45-49 (EXAGEOSTAT_sdmle_Tile,
  * MLE_sdexact.c:172-357): mixed precision.  Here: FP64 diagonal tiles, FP32 /
  * BF16 off-diagonal tiles chosen per theta, FP64 refinement of the solve. */
 double EXAGEOSTAT_sdmle_Tile(unsigned n, const double *theta, double *grad, void *data);
+double EXAGEOSTAT_sdmle_Tile_Async(unsigned n, const double *theta, double *grad, void *data);
+/* exageostat_exact/src/include/MLE_exact_s.h:41-45 (EXAGEOSTAT_smle_Tile,
+ * MLE_sexact.c:155-277): single precision.  Every off-diagonal tile is
+ * stored in FP32 and updated with FP32-accurate products (split BF16 on the
+ * matrix cores); the diagonal tiles, their factorisation and the solve
+ * arithmetic are FP64 (the diagonal-tile kernels are FP64 only); no
+ * refinement.  data->sdotp receives the dot product in FP32. */
+double EXAGEOSTAT_smle_Tile(unsigned n, const double *theta, double *grad, void *data);
+double EXAGEOSTAT_smle_Tile_Async(unsigned n, const double *theta, double *grad, void *data);
+/* exageostat_approx/src/include/MLE_approx.h:41-45 (EXAGEOSTAT_dmle_diag_Tile,
+ * MLE_approx.c:201-365): the band-limited factorisation of
+ * CHAM_pdpotrf_diag (pdpotrf_diag.c:40-110) with band data->diag_thick
+ * tiles, then the log-determinant over the diagonal tiles and the forward
+ * solve over the whole lower triangle (whose tiles outside the band still
+ * hold the covariance), as there. */
+double EXAGEOSTAT_dmle_diag_Tile(unsigned n, const double *theta, double *grad, void *data);
+double EXAGEOSTAT_dmle_diag_Tile_Async(unsigned n, const double *theta, double *grad, void *data);
 
 /* ------------------------------------------------------------ tile tasks -- */
 /* runtime/starpu/include/starpu_exageostat.h:52-55, codelet_zcmg.c:246-347. */
@@ -203,6 +406,20 @@ double EXAGEOSTAT_dmle_Predict_Tile(MLE_data *data, double *theta, int nZmiss, i
                                     double *Zactual, double *Zmiss, int n);
 double MORSE_dmle_Predict_Tile(MLE_data *data, double *theta, int nZmiss, int nZobs, double *Zobs,
                                double *Zactual, double *Zmiss, int n);
+/* MLE_exact.h:55-56 (MLE_dexact.c:876-1000): held-out locations = the first
+ * nZmiss of data->l1, observed = the rest; Zobs and Zactual from
+ * data->descZobs / data->descZactual (set by Predict_Allocate, filled by
+ * the caller with EXAGEOSTAT_Lapack_to_Tile), predictions into
+ * data->descZmiss; returns the MSE (-1 without actual values). */
+double EXAGEOSTAT_dmle_Predict_Tile_Async(MLE_data *data, double *theta, int nZmiss, int nZobs, int n);
+/* MLE_exact_s.h:47-50, MLE_sdexact.h:51-54, MLE_approx.h:47-50: the FP64
+ * predictor serves every path here. */
+double EXAGEOSTAT_smle_Predict_Tile(MLE_data *data, double *theta, int nZmiss, int nZobs, double *Zobs,
+                                    double *Zactual, double *Zmiss, int n);
+double EXAGEOSTAT_sdmle_Predict_Tile(MLE_data *data, double *theta, int nZmiss, int nZobs, double *Zobs,
+                                     double *Zactual, double *Zmiss, int n);
+double EXAGEOSTAT_dmle_diag_Predict_Tile(MLE_data *data, double *theta, int nZmiss, int nZobs, double *Zobs,
+                                         double *Zactual, double *Zmiss, int n);
 /* src/compute/MLE.c:284-306, 341-380. */
 void prediction_init(MLE_data *data, int nZmiss, int nZobs, int ts, int p_grid, int q_grid, int mse_flag);
 void prediction_finalize(MLE_data *data);

@@ -1,0 +1,150 @@
+"""GPU parity of the mixed-precision tile classes that the n = 90,000 and
+n = 160,000 workloads run, against the FP64 oracle.
+
+The fused three-class update (one launch per step) is taken when the tile
+size is a multiple of 192; other tile sizes take one launch per class with
+128 x 128 or 64 x 64 blocks.  These tests pin both against the oracle's
+dense FP64 likelihood: ts = 192 and 384 (fused launch), ts = 1,024 (the
+tile size of the n = 160,000 workload, per-class launches), at the
+benchmark's theta and at the two n = 160,000 thetas, with every tile class
+present.  They also pin kriging with the parsimonious bivariate kernel
+against the oracle's predictor, and the mixed path's refusal to return a
+gradient.
+"""
+import numpy as np
+import pytest
+
+from oracle import datagen, likelihood
+
+pytestmark = pytest.mark.gpu
+
+K = "univariate_matern_stationary"
+MIXED_TOL = 1e-6
+THETAS = [(1.0, 0.1, 0.5), (1.0, 0.03, 1.0), (1.0, 0.3, 0.5)]
+# the benchmark's option set: FP32 tiles, BF16 updates below correlation 0.01
+OPTS = dict(bf16=True, fp32_corr=0.01, ir_steps=3)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import exageostat_amd
+    exageostat_amd.load()
+    assert exageostat_amd.device_count() >= 1, "no HIP device visible"
+    return exageostat_amd
+
+
+_FIELDS = {}
+
+
+def _field(n, theta):
+    """Morton-ordered jittered grid (seed 0) with z drawn from Sigma(theta);
+    the oracle's (l, logdet, dotp) at theta."""
+    key = (n, tuple(theta))
+    if key not in _FIELDS:
+        x, y, _ = datagen.generate_xy_loc(n, 0)
+        e, _ = datagen.dlarnv(3, [0, 0, 0, 1], n)
+        z = likelihood.dzvg(K, x, y, theta, e)
+        _FIELDS[key] = (x, y, z, likelihood.dmle(K, x, y, z, theta))
+    return _FIELDS[key]
+
+
+def _check(lib, n, ts, theta, **extra):
+    x, y, z, (ll, ld, dp) = _field(n, theta)
+    m = lib.ExactMLE(x, y, ts, precision=2)
+    m.set_mixed(**dict(OPTS, **extra))
+    m.set_z(z)
+    val = m.loglik(theta)
+    pm = m.prec_map()
+    status, logdet, dotp = m.status, m.logdet, m.dotp
+    m.close()
+    print(f"mixed n={n} ts={ts} theta={theta} {extra}: rel dl {abs(val - ll) / abs(ll):.2e} "
+          f"dlogdet {logdet - ld:+.2e} rel ddotp {abs(dotp - dp) / abs(dp):.2e}", flush=True)
+    assert status == 0
+    assert abs(val - ll) <= MIXED_TOL * abs(ll), (n, ts, theta, val, ll)
+    # three FP64 refinement steps: the quadratic form to FP64-level accuracy
+    assert abs(dotp - dp) <= 1e-8 * abs(dp), (dotp, dp)
+    assert abs(logdet - ld) <= 2e-6 * abs(ll), (logdet, ld)
+    return pm
+
+
+@pytest.mark.parametrize("ts", [192, 384])
+@pytest.mark.parametrize("theta", THETAS, ids=["c2theta", "c3smooth", "c3rough"])
+def test_fused_update_matches_oracle(lib, ts, theta):
+    """n = 5,760 (30 or 15 tile rows): the fused launch of the benchmark,
+    with the FP64, FP32 and BF16 classes all present at the benchmark's
+    theta."""
+    pm = _check(lib, 5760, ts, theta)
+    low = pm[np.tril_indices(pm.shape[0])]
+    assert np.all(np.diag(pm) == 0)
+    if theta == THETAS[0]:
+        assert np.any(low == 1) and np.any(low == 3), np.unique(low, return_counts=True)
+
+
+@pytest.mark.parametrize("theta", THETAS[1:], ids=["c3smooth", "c3rough"])
+def test_ts1024_matches_oracle(lib, theta):
+    """n = 6,144, ts = 1,024 (the tile size of the n = 160,000 workload)."""
+    _check(lib, 6144, 1024, theta)
+
+
+@pytest.mark.parametrize("ts", [192, 1024])
+def test_fused_update_lolo_matches_oracle(lib, ts):
+    """The four-product FP32 class (lo*lo kept) on both launch shapes."""
+    _check(lib, 6144 if ts == 1024 else 5760, ts, THETAS[2], lolo=True)
+
+
+def test_fused_update_fp64_band_matches_oracle(lib):
+    """Strongly correlated tiles kept in FP64 (fp64_corr) in the fused
+    launch: the FP64 class then also updates off-diagonal tiles from FP32
+    panel tiles."""
+    pm = _check(lib, 5760, 192, THETAS[2], fp64_corr=0.9)
+    low = pm[np.tril_indices(pm.shape[0])]
+    off = pm[np.tril_indices(pm.shape[0], -1)]
+    assert np.any(off == 0) and np.any(low == 1)
+
+
+def test_mixed_gradient_is_refused(lib):
+    """precision 2 with a gradient request: status -1 and an error string,
+    never a NaN gradient with status 0."""
+    import ctypes
+
+    from exageostat_amd import _lib
+    x, y, z, _ = _field(5760, THETAS[0])
+    m = lib.ExactMLE(x[:1600], y[:1600], 320, precision=2)
+    m.set_mixed(**OPTS)
+    m.set_z(z[:1600])
+    th = np.array(THETAS[0])
+    g = np.zeros(3)
+    c = _lib.load()
+    val = c.MLE_alg(3, th.ctypes.data_as(_lib.c_double_p), g.ctypes.data_as(_lib.c_double_p),
+                    ctypes.cast(ctypes.pointer(m.data), ctypes.c_void_p))
+    assert val == -np.inf
+    assert m.data.status == -1
+    assert "gradient" in c.exageostat_gpu_last_error().decode()
+    # the same object still evaluates l without a gradient
+    ll = m.loglik(th)
+    assert np.isfinite(ll) and m.status == 0
+    m.close()
+
+
+def test_bivariate_kriging_matches_oracle(lib):
+    """Parsimonious bivariate kriging at the theta of the 50,000-location
+    workload: 600 observed and 100 held-out locations (200 held-out rows),
+    per-row predictions and the per-variable MSE against the oracle."""
+    kern = "bivariate_matern_parsimonious"
+    theta = np.array([1.0, 1.0, 0.1, 0.5, 1.0, 0.5])
+    nloc = 700
+    x, y, _ = datagen.generate_xy_loc(nloc, 3)
+    e, _ = datagen.dlarnv(3, [3, 3, 3, 1], 2 * nloc)
+    z = likelihood.dzvg(kern, x, y, theta, e).reshape(nloc, 2)
+    perm = np.random.default_rng(5).permutation(nloc)
+    miss, obs = np.sort(perm[:100]), np.sort(perm[100:])
+    zo, za = z[obs].ravel(), z[miss].ravel()
+    ref, mse, mse1, mse2 = likelihood.predict(kern, x[obs], y[obs], zo, x[miss], y[miss], theta, za)
+    m = lib.ExactMLE(x[obs], y[obs], 128, kernel=kern)
+    zm, mse_gpu = m.predict(theta, x[obs], y[obs], zo, x[miss], y[miss], za)
+    d1, d2 = m.data.mserror1, m.data.mserror2
+    m.close()
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(zm - ref)) <= 1e-9 * scale
+    assert abs(mse_gpu - mse) <= 1e-9 * mse
+    assert abs(d1 - mse1) <= 1e-9 * mse1 and abs(d2 - mse2) <= 1e-9 * mse2
