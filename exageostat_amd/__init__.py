@@ -5,4 +5,4 @@ the reference's C entry points, include/exageostat_gpu.h); this package
 loads it with ctypes and mirrors the driver-level interface in ``mle``.
 """
 from ._lib import load, LIB_PATH  # noqa: F401
-from .mle import ExactMLE, DeviceMatrix, generate_xy_loc, device_count  # noqa: F401
+from .mle import ExactMLE, DeviceMatrix, generate_xy_loc, generate_xy_loc_st, device_count  # noqa: F401

@@ -80,6 +80,9 @@ class MLEData(ctypes.Structure):
 
 
 _P = ctypes.POINTER
+_IP = ctypes.POINTER(ctypes.c_int)
+# exageostat_func_t: the NLOPT objective type (MLE_alg's signature)
+OBJECTIVE = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_uint, c_double_p, c_double_p, ctypes.c_void_p)
 _SIGNATURES = {
     "exageostat_init": (None, [_P(ctypes.c_int)] * 4),
     "exageostat_finalize": (None, []),
@@ -87,6 +90,7 @@ _SIGNATURES = {
     "MLE_Finalize": (None, [_P(MLEData)]),
     "MLE_get_zobs": (None, [_P(MLEData), c_double_p, ctypes.c_int]),
     "GenerateXYLoc": (_P(Location), [ctypes.c_int, ctypes.c_int]),
+    "GenerateXYLoc_ST": (_P(Location), [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pick_random_points": (None, [_P(MLEData), c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "MLE_zvg": (None, [_P(MLEData), c_double_p, c_double_p] + [ctypes.c_int] * 5),
     "EXAGEOSTAT_MLE_dzvg_Tile": (None, [_P(MLEData), c_double_p, c_double_p] + [ctypes.c_int] * 3),
@@ -138,6 +142,26 @@ _SIGNATURES = {
     "EXAGEOSTAT_dpotri_Tile": (ctypes.c_int, [ctypes.c_int, _P(Desc)]),
     "EXAGEOSTAT_dtrsv_Tile": (ctypes.c_int, [ctypes.c_int, _P(Desc), c_double_p]),
     "exageostat_dlarnv": (None, [ctypes.c_int, _P(ctypes.c_int), ctypes.c_int, c_double_p]),
+    "exageostat_bobyqa": (ctypes.c_int, [ctypes.c_uint, OBJECTIVE, ctypes.c_void_p, c_double_p, c_double_p,
+                                         c_double_p, ctypes.c_double, ctypes.c_int, c_double_p,
+                                         _P(ctypes.c_int)]),
+    "init_data_values": (None, [_P(MLEData)]),
+    "readLocsFile": (_P(Location), [ctypes.c_char_p, ctypes.c_int]),
+    "readLocsFile3d": (_P(Location), [ctypes.c_char_p, ctypes.c_int]),
+    "readObsFile": (c_double_p, [ctypes.c_char_p, ctypes.c_int]),
+    "readTimeFile": (c_double_p, [ctypes.c_char_p, ctypes.c_int]),
+    "rexageostat_init": (None, [_IP] * 4),
+    "rexageostat_finalize": (None, []),
+    "gen_z_givenlocs_exact": (None, [c_double_p, _IP, c_double_p, _IP, _IP, c_double_p] + [_IP] * 9 + [c_double_p]),
+    "gen_z_exact": (None, [_IP, c_double_p] + [_IP] * 10 + [c_double_p]),
+    "mle_exact": (None, [c_double_p, _IP] * 5 + [_IP] * 3 + [c_double_p] + [_IP] * 6 + [c_double_p]),
+    "mle_dst": (None, [c_double_p, _IP] * 3 + [_IP, c_double_p, _IP, c_double_p, _IP, _IP, _IP, _IP, c_double_p]
+                + [_IP] * 6 + [c_double_p]),
+    "mle_mp": (None, [c_double_p, _IP] * 3 + [_IP, c_double_p, _IP, c_double_p, _IP, _IP, _IP, _IP, c_double_p]
+               + [_IP] * 6 + [c_double_p]),
+    "exact_predict": (None, [c_double_p, _IP] * 5 + [_IP, _IP, _IP, c_double_p] + [_IP] * 8 + [c_double_p]),
+    "exact_mloe_mmom": (None, [c_double_p, _IP] * 5 + [_IP, _IP, _IP, c_double_p, c_double_p] + [_IP] * 7
+                        + [c_double_p]),
     "exageostat_gpu_synchronize": (None, []),
     "exageostat_gpu_profile_enable": (None, [ctypes.c_int]),
     "exageostat_gpu_profile_reset": (None, []),
@@ -168,7 +192,7 @@ def header_symbols(path: str = HEADER_PATH) -> list[str]:
     """Function names declared in include/exageostat_gpu.h."""
     text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M)
+    names = re.findall(r"^(?!typedef)[A-Za-z_][\w \*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M)
     return sorted(set(n for n in names if n not in ("if", "defined")))
 
 

@@ -444,8 +444,15 @@ double evaluate(MLE_data* data, unsigned n, const double* theta, double* grad, P
         // Profile likelihood of the "_profile" kernels (MLE_dexact.c:403-421),
         // N/2 in integer arithmetic as there; variance_v = |(L^-1 z)_v|^2 / (N/2).
         const long half = N / 2;
-        loglik = -(double)half + (double)half * std::log((double)N) - (double)half * std::log(dotp) -
-                 0.5 * logdet - (N / 2.0) * std::log(2.0 * M_PI);
+        // The band-limited path writes the interleaved kernel's profile with
+        // (N/2.0) log(N/2.0) in place of (N/2) log N (MLE_approx.c:386-396).
+        const bool band_form = path == Path::kBand && kid == kBivariate;
+        if (band_form)
+          loglik = -(N / 2.0) + (N / 2.0) * std::log(N / 2.0) - (N / 2.0) * std::log(dotp) - 0.5 * logdet -
+                   (N / 2.0) * std::log(2.0 * M_PI);
+        else
+          loglik = -(double)half + (double)half * std::log((double)N) - (double)half * std::log(dotp) -
+                   0.5 * logdet - (N / 2.0) * std::log(2.0 * M_PI);
         data->dotp1 = data->dotp2 = std::nan("");
         if (path == Path::kExact || path == Path::kBand) {
           std::vector<double> w(N);
@@ -876,6 +883,36 @@ location* GenerateXYLoc(int n, int seed) {
     loc->x[i] = (double)compact1by1(code[i]) / 65535.0;
     loc->y[i] = (double)compact1by1(code[i] >> 1) / 65535.0;
   }
+  return loc;
+}
+
+location* GenerateXYLoc_ST(int n, int t_slots, int seed) {
+  srand(seed);
+  const long total = (long)n * (t_slots > 0 ? t_slots : 0);
+  auto* loc = static_cast<location*>(malloc(sizeof(location)));
+  loc->x = static_cast<double*>(malloc(sizeof(double) * (total > 0 ? total : 1)));
+  loc->y = static_cast<double*>(malloc(sizeof(double) * (total > 0 ? total : 1)));
+  loc->z = static_cast<double*>(malloc(sizeof(double) * (total > 0 ? total : 1)));
+  if (t_slots <= 0) return loc;
+  const int sqrtn = (int)ceil(sqrt((double)n));
+  int index = 0;
+  for (int i = 0; i < sqrtn && index < n; ++i)
+    for (int j = 0; j < sqrtn && index < n; ++j) {
+      const double ux = (double)rand() / (1.0 + RAND_MAX) * 0.8 - 0.4;
+      loc->x[index] = ((i + 1) - 0.5 + ux) / sqrtn;
+      const double uy = (double)rand() / (1.0 + RAND_MAX) * 0.8 - 0.4;
+      loc->y[index] = ((j + 1) - 0.5 + uy) / sqrtn;
+      loc->z[index] = 1.0;
+      ++index;
+    }
+  // No Morton sort: the spatial points of slot 0 repeat in every slot, with
+  // time slot + 1.
+  for (int t = 1; t < t_slots; ++t)
+    for (int i = 0; i < n; ++i) {
+      loc->x[i + (long)t * n] = loc->x[i];
+      loc->y[i + (long)t * n] = loc->y[i];
+      loc->z[i + (long)t * n] = (double)(t + 1);
+    }
   return loc;
 }
 

@@ -42,6 +42,7 @@ struct MixedState {
   const double* bbox_y = nullptr;
   long bbox_n = 0;
   int bbox_ts = 0;
+  int bbox_layout = -1;
   ~MixedState() {
     if (part) (void)hipFree(part);
     if (split) (void)hipFree(split);
@@ -50,19 +51,27 @@ struct MixedState {
 
 void destroy_mixed(MixedState* m) { delete m; }
 
-// Per-tile bounding boxes: tile t covers rows [t*ts, (t+1)*ts); row i sits at
-// location i / vars.
-static void tile_bboxes(MixedState& st, const double* x, const double* y, uint64_t fp, long nloc, int vars,
+// Row -> location layouts of the covariance rows.
+enum RowLayout { kRowPlain = 0, kRowInterleaved = 1, kRowBlock = 2 };
+
+// Location of covariance row i: i (one variable), i / 2 (two variables
+// interleaved by row parity), i mod nloc (variable 1 rows, then variable 2).
+static inline long row_location(long i, int layout, long nloc) {
+  return layout == kRowInterleaved ? i / 2 : (layout == kRowBlock ? (i < nloc ? i : i - nloc) : i);
+}
+
+// Per-tile bounding boxes: tile t covers rows [t*ts, (t+1)*ts).
+static void tile_bboxes(MixedState& st, const double* x, const double* y, uint64_t fp, long nloc, int layout,
                         long nrows, int ts, int mt) {
   if (st.bbox_x == x && st.bbox_y == y && st.bbox_fp == fp && st.bbox_n == nloc && st.bbox_ts == ts &&
-      (long)st.bbox.size() == 4L * mt)
+      st.bbox_layout == layout && (long)st.bbox.size() == 4L * mt)
     return;
   st.bbox.assign(4L * mt, 0.0);
   for (int t = 0; t < mt; ++t) {
     double x0 = HUGE_VAL, x1 = -HUGE_VAL, y0 = HUGE_VAL, y1 = -HUGE_VAL;
     const long lo = (long)t * ts, hi = std::min<long>(nrows, lo + ts);
     for (long i = lo; i < hi; ++i) {
-      const long p = i / vars;
+      const long p = row_location(i, layout, nloc);
       x0 = std::min(x0, x[p]);
       x1 = std::max(x1, x[p]);
       y0 = std::min(y0, y[p]);
@@ -78,6 +87,7 @@ static void tile_bboxes(MixedState& st, const double* x, const double* y, uint64
   st.bbox_fp = fp;
   st.bbox_n = nloc;
   st.bbox_ts = ts;
+  st.bbox_layout = layout;
 }
 
 // Largest correlation any entry of the Matern kernel with smoothness nu and
@@ -93,12 +103,13 @@ static double corr_bound(double d, double beta, const std::vector<KnuOrder<doubl
 // Precision class of every tile of the lower factor for this theta.
 static std::vector<uint8_t> precision_map(MixedState& st, const Context& ctx, KernelId kid, const double* theta,
                                           int metric, const MixedOptions& opt, int mt, int ts) {
-  const int vars = kid == kBivariate ? 2 : 1;
+  const bool two = kid == kBivariate || kid == kBivariateBlock;
+  const int layout = kid == kBivariate ? kRowInterleaved : (kid == kBivariateBlock ? kRowBlock : kRowPlain);
   std::vector<KnuOrder<double>> ko;
   std::vector<double> con;
   std::vector<double> nus;
   double beta;
-  if (kid == kBivariate) {
+  if (two) {
     beta = theta[2];
     nus = {theta[3], theta[4], 0.5 * (theta[3] + theta[4])};
   } else {
@@ -111,8 +122,11 @@ static std::vector<uint8_t> precision_map(MixedState& st, const Context& ctx, Ke
     ko.push_back(o);
     con.push_back(1.0 / (std::pow(2.0, nu - 1.0) * std::tgamma(nu)));
   }
-  const bool boxes = metric == 0 && ctx.l1.hx && ctx.l1.hy;
-  if (boxes) tile_bboxes(st, ctx.l1.hx, ctx.l1.hy, ctx.l1.fp, ctx.l1.n, vars, ctx.N, ts, mt);
+  // Box bounds need Euclidean 2-D distances and a kernel that decreases
+  // with them; the space-time kernel also depends on the time lag, and
+  // great-circle or 3-D distances have no box bound here.
+  const bool boxes = metric == 0 && ctx.l1.hx && ctx.l1.hy && !ctx.l1.hz && kid != kSpaceTime;
+  if (boxes) tile_bboxes(st, ctx.l1.hx, ctx.l1.hy, ctx.l1.fp, ctx.l1.n, layout, ctx.N, ts, mt);
   std::vector<uint8_t> prec((size_t)mt * (mt + 1) / 2);
   const int thick = std::max(1, opt.diag_thick);
   long idx = 0;
@@ -130,8 +144,8 @@ static std::vector<uint8_t> precision_map(MixedState& st, const Context& ctx, Ke
         if (opt.fp64_corr > 0.0 && corr >= opt.fp64_corr) c = kF64;
         else if (opt.bf16 && corr < opt.fp32_corr) c = kF32B;
       } else if (opt.fp64_corr > 0.0) {
-        // Great-circle distances: no box bound, every off-band tile is
-        // treated as strongly correlated.
+        // No box bound: every off-band tile is treated as strongly
+        // correlated.
         c = kF64;
       }
       prec[idx] = c;

@@ -18,6 +18,10 @@ KERNEL_PARAMS = {
     "univariate_matern_stationary": 3,
     "univariate_matern_nuggets_stationary": 4,
     "bivariate_matern_parsimonious": 6,
+    "bivariate_matern_parsimonious_profile": 6,
+    "bivariate_matern_parsimonious2": 6,
+    "bivariate_matern_parsimonious2_profile": 6,
+    "univariate_spacetime_matern_stationary": 7,
 }
 
 
@@ -48,6 +52,20 @@ def generate_xy_loc(n: int, seed: int):
     libc.free(ctypes.cast(loc.contents.y, ctypes.c_void_p))
     libc.free(ctypes.cast(loc, ctypes.c_void_p))
     return x, y
+
+
+def generate_xy_loc_st(n: int, t_slots: int, seed: int):
+    """GenerateXYLoc_ST through the library: (x, y, t), n points per slot."""
+    lib = _lib.load()
+    loc = lib.GenerateXYLoc_ST(n, t_slots, seed)
+    total = n * t_slots
+    out = [np.ctypeslib.as_array(getattr(loc.contents, f), shape=(total,)).copy() for f in ("x", "y", "z")]
+    libc = ctypes.CDLL("libc.so.6")
+    libc.free.argtypes = [ctypes.c_void_p]
+    for f in ("x", "y", "z"):
+        libc.free(ctypes.cast(getattr(loc.contents, f), ctypes.c_void_p))
+    libc.free(ctypes.cast(loc, ctypes.c_void_p))
+    return tuple(out)
 
 
 def dlarnv(idist: int, iseed, n: int):
@@ -82,10 +100,15 @@ def profile_get(name: str) -> dict:
 
 class ExactMLE:
     """One MLE_data with its GPU context: EXAGEOSTAT_dmle_Call at
-    construction, MLE_Finalize at close()."""
+    construction, MLE_Finalize at close().
+
+    t: third coordinate of every location (time for the space-time kernel),
+    or None.  computation: "exact" or "diag_approx" (the band-limited
+    factorisation, band = diag_thick tiles)."""
 
     def __init__(self, x, y, ts: int, kernel: str = "univariate_matern_stationary", dm: str = "ed",
-                 precision: int = 0, silent: bool = True):
+                 precision: int = 0, silent: bool = True, t=None, computation: str = "exact",
+                 diag_thick: int = 1):
         self.lib = _lib.load()
         if kernel not in KERNEL_PARAMS:
             raise ValueError(f"unsupported kernel {kernel!r}")
@@ -93,10 +116,11 @@ class ExactMLE:
         self.nparams = KERNEL_PARAMS[kernel]
         self.x = np.ascontiguousarray(x, dtype=np.float64)
         self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.t = None if t is None else np.ascontiguousarray(t, dtype=np.float64)
         self.nloc = len(self.x)
         self.N = self.nloc * (2 if kernel.startswith("bivariate") else 1)
         self.ts = ts
-        self._strings = [b"exact", kernel.encode(), dm.encode(), b""]
+        self._strings = [computation.encode(), kernel.encode(), dm.encode(), b""]
         d = MLEData()
         d.computation = self._strings[0]
         d.kernel_fun = self._strings[1]
@@ -106,8 +130,8 @@ class ExactMLE:
         d.silent = 1 if silent else 0
         d.l1.x = _dp(self.x)
         d.l1.y = _dp(self.y)
-        d.l1.z = None
-        d.diag_thick = 1
+        d.l1.z = _dp(self.t) if self.t is not None else None
+        d.diag_thick = diag_thick
         d.mp_ir_steps = 3
         self.data = d
         zero = ctypes.c_int(0)
@@ -206,6 +230,11 @@ class ExactMLE:
         return self.data.logdet
 
     @property
+    def variances(self):
+        """(variance1, variance2) of the last profile-kernel evaluation."""
+        return self.data.variance1, self.data.variance2
+
+    @property
     def dotp(self) -> float:
         return self.data.dotp
 
@@ -230,8 +259,9 @@ class ExactMLE:
         return F, Finv
 
     # -- prediction ------------------------------------------------------
-    def predict(self, theta, xobs, yobs, zobs, xmiss, ymiss, zactual=None):
-        """EXAGEOSTAT_dmle_Predict_Tile; returns (zmiss, mse)."""
+    def predict(self, theta, xobs, yobs, zobs, xmiss, ymiss, zactual=None, tobs=None, tmiss=None):
+        """EXAGEOSTAT_dmle_Predict_Tile; returns (zmiss, mse).  tobs/tmiss:
+        third coordinates (time) of the observed and held-out locations."""
         xo = np.ascontiguousarray(xobs, dtype=np.float64)
         yo = np.ascontiguousarray(yobs, dtype=np.float64)
         xm = np.ascontiguousarray(xmiss, dtype=np.float64)
@@ -242,10 +272,12 @@ class ExactMLE:
         n_obs = len(xo) * rows
         zm = np.zeros(n_miss)
         za = None if zactual is None else np.ascontiguousarray(zactual, dtype=np.float64)
-        self._pred_refs = [xo, yo, xm, ym]
+        to = None if tobs is None else np.ascontiguousarray(tobs, dtype=np.float64)
+        tm = None if tmiss is None else np.ascontiguousarray(tmiss, dtype=np.float64)
+        self._pred_refs = [xo, yo, xm, ym, to, tm]
         d = self.data
-        d.lobs.x, d.lobs.y, d.lobs.z = _dp(xo), _dp(yo), None
-        d.lmiss.x, d.lmiss.y, d.lmiss.z = _dp(xm), _dp(ym), None
+        d.lobs.x, d.lobs.y, d.lobs.z = _dp(xo), _dp(yo), (_dp(to) if to is not None else None)
+        d.lmiss.x, d.lmiss.y, d.lmiss.z = _dp(xm), _dp(ym), (_dp(tm) if tm is not None else None)
         th = np.ascontiguousarray(theta, dtype=np.float64)
         self.lib.EXAGEOSTAT_dmle_Predict_Allocate(ctypes.byref(d), len(xm), len(xo), self.ts, 1, 1, 1)
         _check(self.lib, "Predict_Allocate")
@@ -305,15 +337,20 @@ class DeviceMatrix:
         self.m, self.n, self.ts, self.lower = m, n, ts, lower
         self._keep = []
 
-    def dcmg(self, theta, x1, y1, x2=None, y2=None, kernel="univariate_matern_stationary", dm="ed"):
+    def dcmg(self, theta, x1, y1, x2=None, y2=None, kernel="univariate_matern_stationary", dm="ed", t1=None,
+             t2=None):
+        """EXAGEOSTAT_MLE_dcmg_Tile_Async; t1/t2: third coordinates (time
+        for the space-time kernel) of the row and column locations."""
         x1 = np.ascontiguousarray(x1, dtype=np.float64)
         y1 = np.ascontiguousarray(y1, dtype=np.float64)
         x2 = x1 if x2 is None else np.ascontiguousarray(x2, dtype=np.float64)
         y2 = y1 if y2 is None else np.ascontiguousarray(y2, dtype=np.float64)
-        l1 = _lib.Location(_dp(x1), _dp(y1), None)
-        l2 = _lib.Location(_dp(x2), _dp(y2), None)
+        t1 = None if t1 is None else np.ascontiguousarray(t1, dtype=np.float64)
+        t2 = t1 if t2 is None else np.ascontiguousarray(t2, dtype=np.float64)
+        l1 = _lib.Location(_dp(x1), _dp(y1), _dp(t1) if t1 is not None else None)
+        l2 = _lib.Location(_dp(x2), _dp(y2), _dp(t2) if t2 is not None else None)
         th = np.ascontiguousarray(theta, dtype=np.float64)
-        self._keep = [x1, y1, x2, y2, th]
+        self._keep = [x1, y1, x2, y2, th, t1, t2]
         st = self.lib.EXAGEOSTAT_MLE_dcmg_Tile_Async(_lib.ChamLower, self.desc, ctypes.byref(l1), ctypes.byref(l2),
                                                      None, _dp(th), dm.encode(), kernel.encode(), None, None)
         _check(self.lib, "dcmg")

@@ -5,24 +5,33 @@ The reference maximises l(theta) with NLOPT's derivative-free BOBYQA
 at lb, stopping on an absolute change of l below 10^-opt_tol
 (misc/compute/MLE_misc.c:1352-1366) or after opt_max_iters evaluations.
 
-NLOPT is not available to this package, so two drivers are provided:
+NLOPT is not available to this package, so three drivers are provided:
 
-* ``fit(..., method="dfo")``: the derivative-free comparator, scipy's COBYQA
-  (Ragonneau & Zhang: a trust-region method on quadratic interpolation
-  models, the bound-constrained successor of Powell's BOBYQA) with the
+* ``fit(..., method="bobyqa")``: the reference's run with the library's own
+  BOBYQA (``exageostat_bobyqa``, Powell 2009, with NLOPT's conventions:
+  2n+1 points, default initial step, start at lb, stop when an evaluation
+  is within 10^-opt_tol of the best or after max_evals).  For an ExactMLE
+  the objective handed to it is the C function MLE_alg itself with the
+  MLE_data pointer, as the reference hands MLE_alg to NLOPT
+  (synthetic_dmle_test.c:283), so no Python runs inside the loop;
+* ``fit(..., method="dfo")``: scipy's COBYQA (Ragonneau & Zhang, a
+  trust-region method on quadratic interpolation models) with the
   reference's bounds, start and evaluation budget;
 * ``fit(..., method="lbfgsb")``: L-BFGS-B driven by the adjoint gradient that
   MLE_alg returns in its grad argument.
 
-Both evaluate l through the same C entry point as the reference's optimiser.
+All evaluate l through the same C entry point as the reference's optimiser.
 """
 from __future__ import annotations
 
+import ctypes
 import dataclasses
 import time
 
 import numpy as np
 from scipy import optimize
+
+from . import _lib
 
 # l is -inf (and the gradient NaN) where Sigma(theta) is not positive
 # definite; the minimisers get this finite penalty instead.
@@ -46,6 +55,9 @@ def fit(m, lb, ub, start=None, method: str = "lbfgsb", opt_tol: int = 5, max_eva
     """Maximise l(theta) over the box [lb, ub] for the data held by the
     ExactMLE ``m`` (precision 0 or 2).
 
+    method "bobyqa": the library's BOBYQA, derivative-free, stops when an
+    evaluation lies within 10^-opt_tol of the best value (NLOPT's ftol_abs,
+    as the reference sets it) or after max_evals evaluations.
     method "dfo": COBYQA, derivative-free, stops when the trust region falls
     below 10^-(opt_tol+1) of the bound box or after max_evals evaluations.
     method "lbfgsb": L-BFGS-B with the adjoint gradient, stops when the
@@ -107,7 +119,39 @@ def fit(m, lb, ub, start=None, method: str = "lbfgsb", opt_tol: int = 5, max_eva
         return f, -np.asarray(g) * jac_scale(th) / n
 
     t0 = time.perf_counter()
-    if method == "dfo":
+    if method == "bobyqa":
+        lib = _lib.load()
+        k = len(lb)
+        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        lbc = np.ascontiguousarray(lb, dtype=np.float64)
+        ubc = np.ascontiguousarray(ub, dtype=np.float64)
+        fmax = ctypes.c_double(0.0)
+        nev = ctypes.c_int(0)
+        if getattr(m, "data", None) is not None and hasattr(m, "lib"):
+            # the C objective: MLE_alg(n, theta, grad, &data)
+            fn = _lib.OBJECTIVE(ctypes.cast(lib.MLE_alg, ctypes.c_void_p).value)
+            fdata = ctypes.cast(ctypes.pointer(m.data), ctypes.c_void_p)
+        else:
+            def pyobj(nn, xp, gp, _d):
+                th = np.array([xp[i] for i in range(nn)])
+                val = float(m.loglik(th))
+                history.append((th, val))
+                return val if np.isfinite(val) else -np.inf
+            fn = _lib.OBJECTIVE(pyobj)
+            fdata = None
+        rc = lib.exageostat_bobyqa(k, fn, fdata, lbc.ctypes.data_as(_lib.c_double_p),
+                                   ubc.ctypes.data_as(_lib.c_double_p), x.ctypes.data_as(_lib.c_double_p),
+                                   10.0 ** (-opt_tol), int(max_evals), ctypes.byref(fmax), ctypes.byref(nev))
+        if rc < 0:
+            raise RuntimeError(f"exageostat_bobyqa failed ({rc})")
+        counts["evals"] = nev.value
+        if not history:
+            history.append((x.copy(), fmax.value))
+
+        class _Res:
+            message = {0: "ftol_abs reached", 1: "max_evals reached", 2: "trust region at its floor"}[rc]
+        res = _Res()
+    elif method == "dfo":
         # Variables scaled to the bound box (BOBYQA's steps are relative to
         # the box as well); stop when the trust region is below
         # 10^-(opt_tol+1) of the box.

@@ -331,6 +331,10 @@ void MLE_get_zobs(MLE_data *data, double *z, int n);
 /* ------------------------------------------------------ synthetic data -- */
 /* misc/compute/MLE_misc.c:763-792 (srand(seed), jittered grid, Morton sort). */
 location *GenerateXYLoc(int n, int seed);
+/* misc/compute/MLE_misc.c:887-923: n jittered-grid points (srand(seed), x
+ * drawn before y, no Morton sort) repeated for t_slots time slots; location
+ * i + t n has z = t + 1 (the time of the space-time kernel). */
+location *GenerateXYLoc_ST(int n, int t_slots, int seed);
 /* misc/compute/MLE_misc.c:1700-1741. */
 void pick_random_points(MLE_data *data, double *Zobs, double *Zactual, int nZmiss, int nZobs, int N);
 /* src/compute/MLE.c:38-92 -> EXAGEOSTAT_MLE_dzvg_Tile, MLE_dexact.c:25-148:
@@ -493,6 +497,81 @@ int EXAGEOSTAT_dtrsv_Tile(CHAM_enum trans, CHAM_desc_t *L, double *b);
 /* LAPACK dlarnv (the reference draws N(0,1) with LAPACKE_dlarnv(3, iseed, N,
  * Nrand), examples/synthetic_dmle_test.c:111,147); iseed is advanced. */
 void exageostat_dlarnv(int idist, int *iseed, int n, double *x);
+
+/* ------------------------------------------------------------- optimiser -- */
+/* The objective type the reference registers with NLOPT (nlopt_func;
+ * MLE_alg has this signature). */
+typedef double exageostat_func_t(unsigned n, const double *x, double *grad, void *data);
+/* Replaces the reference's NLOPT_LN_BOBYQA run (examples/
+ * synthetic_dmle_test.c:140-142,283-284; init_optimizer, misc/compute/
+ * MLE_misc.c:1352-1366), which needs libnlopt: maximises f over the box
+ * [lb, ub] from x (overwritten with the best point) by Powell's BOBYQA
+ * (2009) with 2n+1 interpolation points, NLOPT's default initial step and
+ * variable scaling, variables with lb == ub held fixed; stops when an
+ * evaluation after the initial ones lies within ftol_abs of the best value
+ * (nlopt_set_ftol_abs), after maxeval evaluations (maxeval <= 0: no limit,
+ * as NLOPT), or when the trust region has shrunk to 1e-14 of the initial
+ * step.  *fmax: best value, *nevals: evaluations.  Returns 0 (ftol), 1
+ * (maxeval), 2 (step floor) or < 0 (invalid arguments, degenerate points). */
+int exageostat_bobyqa(unsigned n, exageostat_func_t *f, void *f_data, const double *lb, const double *ub, double *x,
+                      double ftol_abs, int maxeval, double *fmax, int *nevals);
+
+/* ------------------------------------------------- real data and R .C -- */
+/* misc/compute/MLE_misc.c:24-144: MLE_data fields to their defaults (this
+ * implementation's fields included). */
+void init_data_values(MLE_data *data);
+/* misc/compute/flat_file.c:23-63, 65-111: "x,y" (or "x,y,z") per line, n
+ * locations; NULL if the file cannot be opened (the 3-D reader of the
+ * reference exits).  The caller frees x, y (z) and the struct. */
+location *readLocsFile(char *locs_file, int n);
+location *readLocsFile3d(char *locs_file, int n);
+/* flat_file.c:115-145, 147-175: one value per line; malloc'd array of n
+ * doubles, or NULL if the file cannot be opened (the reference exits). */
+double *readObsFile(char *obsfile, int n);
+double *readTimeFile(char *timefile, int n);
+/* The R package's .C entry points (r-wrappers/compute/rwrappers.c), every
+ * argument a pointer.  Kernels: 0 univariate Matern, 1 with nugget, 3
+ * parsimonious bivariate, 5 space-time (generation only; 2, 4 and 6 are
+ * not supported).  dmetric 0 Euclidean, 1 great circle.  The maximum-
+ * likelihood calls run exageostat_bobyqa over MLE_alg from clb with
+ * ftol_abs = *opt_tol and at most *opt_max_iters evaluations (<= 0: no
+ * limit) and write [theta hat, seconds per evaluation, total seconds,
+ * evaluations] to globalthetaout (theta hat -1 on failure).  :1172-1187. */
+void rexageostat_init(int *ncores, int *gpus, int *dts, int *lts);
+void rexageostat_finalize(void);
+/* :56-176: z = L(theta) e at the given locations, e = dlarnv(3, {0,0,0,1});
+ * globalvec (veclen >= rows) receives z. */
+void gen_z_givenlocs_exact(double *x, int *xlen, double *y, int *ylen, int *kernel, double *theta, int *thetalen,
+                           int *dmetric, int *n, int *ncores, int *gpus, int *ts, int *p_grid, int *q_grid,
+                           int *veclen, double *globalvec);
+/* :178-301: GenerateXYLoc(n, seed), e = dlarnv(3, {seed,seed,seed,1});
+ * globalvec = [x (n), y (n), z (rows)]. */
+void gen_z_exact(int *kernel, double *theta, int *thetalen, int *dmetric, int *n, int *seed, int *ncores, int *gpus,
+                 int *ts, int *p_grid, int *q_grid, int *veclen, double *globalvec);
+/* :710-753 (computation "exact", precision 0). */
+void mle_exact(double *x, int *xlen, double *y, int *ylen, double *z, int *zlen, double *clb, int *clblen, double *cub,
+               int *cublen, int *kernel, int *dmetric, int *n, double *opt_tol, int *opt_max_iters, int *ncores,
+               int *gpus, int *ts, int *p_grid, int *q_grid, double *globalthetaout);
+/* :848-891 ("diag_approx" with band dst_thick). */
+void mle_dst(double *x, int *xlen, double *y, int *ylen, double *z, int *zlen, int *kernel, double *clb, int *clblen,
+             double *cub, int *cublen, int *dst_thick, int *dmetric, int *n, double *opt_tol, int *opt_max_iters,
+             int *ncores, int *gpus, int *ts, int *p_grid, int *q_grid, double *globalthetaout);
+/* :893-936 (mixed precision, FP64 band mp_band; the reference evaluates
+ * these in FP64 because its mle_general leaves precision at 0). */
+void mle_mp(double *x, int *xlen, double *y, int *ylen, double *z, int *zlen, int *kernel, double *clb, int *clblen,
+            double *cub, int *cublen, int *mp_band, int *dmetric, int *n, double *opt_tol, int *opt_max_iters,
+            int *ncores, int *gpus, int *ts, int *p_grid, int *q_grid, double *globalthetaout);
+/* :938-1072: kriging; globalzpredict receives nZmiss (x2 for bivariate)
+ * predictions (-1000 where nothing was computed); computation must be 0. */
+void exact_predict(double *xobs, int *xobs_len, double *yobs, int *yobs_len, double *zobs, int *zobs_len,
+                   double *xmiss, int *xmiss_len, double *ymiss, int *ymiss_len, int *nZobs, int *nZmiss, int *kernel,
+                   double *est_theta, int *thetalen, int *computation, int *dmetric, int *ncores, int *gpus, int *ts,
+                   int *p_grid, int *q_grid, double *globalzpredict);
+/* :1074-1170: global_mloe_mmom = [MLOE, MMOM] (-1000 on failure). */
+void exact_mloe_mmom(double *xobs, int *xobs_len, double *yobs, int *yobs_len, double *zobs, int *zobs_len,
+                     double *xmiss, int *xmiss_len, double *ymiss, int *ymiss_len, int *nZobs, int *nZmiss,
+                     int *kernel, double *est_theta, double *true_theta, int *thetalen, int *dmetric, int *ncores,
+                     int *gpus, int *ts, int *p_grid, int *q_grid, double *global_mloe_mmom);
 
 /* ------------------------------------------------------------ measurement -- */
 /* No counterpart in the reference.  Per-kernel HIP-event timing on the

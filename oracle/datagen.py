@@ -149,6 +149,32 @@ def generate_xy_loc(n: int, seed: int, rng: GlibcRand | None = None):
     return x, y, rng
 
 
+def generate_xy_loc_st(n: int, t_slots: int, seed: int, rng: GlibcRand | None = None):
+    """GenerateXYLoc_ST (misc/compute/MLE_misc.c:887-923): the jittered grid
+    of generate_xy_loc (x drawn before y, truncated at n) without the Morton
+    sort, repeated for every time slot: location i + j n is spatial point i
+    at time z = j + 1 (j = 0 .. t_slots - 1).  Returns (x, y, z, rng)."""
+    if rng is None:
+        rng = GlibcRand(seed)
+    else:
+        rng.srand(seed)
+    sqrtn = int(math.ceil(math.sqrt(n)))
+    x = np.empty(n)
+    y = np.empty(n)
+    idx = 0
+    for i in range(sqrtn):
+        if idx >= n:
+            break
+        for j in range(sqrtn):
+            if idx >= n:
+                break
+            x[idx] = ((i + 1) - 0.5 + uniform_distribution(rng, -0.4, 0.4)) / sqrtn
+            y[idx] = ((j + 1) - 0.5 + uniform_distribution(rng, -0.4, 0.4)) / sqrtn
+            idx += 1
+    z = np.repeat(np.arange(1.0, t_slots + 1.0), n)
+    return np.tile(x, t_slots), np.tile(y, t_slots), z, rng
+
+
 _TWOPI = 6.28318530717958647692528676655900576839
 _DLARUV_A = 33952834046453
 _M48 = (1 << 48) - 1

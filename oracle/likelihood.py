@@ -28,15 +28,16 @@ from . import matern
 LOG_2PI = math.log(2.0 * math.pi)
 
 
-def sigma(kernel: str, x, y, theta, metric: int = 0) -> np.ndarray:
-    """Full symmetric covariance of one location set."""
-    return matern.cov_matrix(kernel, x, y, x, y, theta, metric)
+def sigma(kernel: str, x, y, theta, metric: int = 0, t=None) -> np.ndarray:
+    """Full symmetric covariance of one location set (t: time coordinates
+    of the space-time kernel)."""
+    return matern.cov_matrix(kernel, x, y, x, y, theta, metric, t1=t, t2=t)
 
 
-def dzvg(kernel: str, x, y, theta, e, metric: int = 0) -> np.ndarray:
+def dzvg(kernel: str, x, y, theta, e, metric: int = 0, t=None) -> np.ndarray:
     """EXAGEOSTAT_MLE_dzvg_Tile (MLE_dexact.c:25-148): z = L(theta) e with
     L the lower Cholesky factor of Sigma(theta) (dcmg -> dpotrf -> dtrmm)."""
-    L = np.linalg.cholesky(sigma(kernel, x, y, theta, metric))
+    L = np.linalg.cholesky(sigma(kernel, x, y, theta, metric, t))
     return L @ np.asarray(e, dtype=np.float64)
 
 
@@ -54,13 +55,99 @@ def loglik_from_sigma(S: np.ndarray, z: np.ndarray):
     return -0.5 * dotp - 0.5 * logdet - (n / 2.0) * LOG_2PI, logdet, dotp
 
 
-def dmle(kernel: str, x, y, z, theta, metric: int = 0):
+def dmle(kernel: str, x, y, z, theta, metric: int = 0, t=None):
     """EXAGEOSTAT_dmle_Tile (MLE_dexact.c:231-489), dense.
     Returns (l, logdet, dotp)."""
-    return loglik_from_sigma(sigma(kernel, x, y, theta, metric), z)
+    return loglik_from_sigma(sigma(kernel, x, y, theta, metric, t), z)
 
 
-def predict(kernel: str, xobs, yobs, zobs, xmiss, ymiss, theta, zactual=None, metric: int = 0):
+def profile_from_solve(kernel: str, v: np.ndarray, logdet: float, approx: bool = False):
+    """The profile likelihood of the "_profile" bivariate kernels from
+    v = L^-1 z and log|Sigma| (exageostat_exact/src/compute/
+    MLE_dexact.c:403-421): with N/2 in integer arithmetic as there,
+      l = -(N/2) + (N/2) log N - (N/2) log(v.v) - logdet/2 - (N/2.0) log 2 pi,
+    and variance_a = |v_a|^2 / (N/2), v_a the rows of variable a (the first
+    and second half of v in the block layout of
+    "bivariate_matern_parsimonious2_profile", even and odd rows in the
+    interleaved layout).  approx: the band-limited path's formula for the
+    interleaved kernel (exageostat_approx/src/compute/MLE_approx.c:386-396),
+    which has (N/2.0) log(N/2.0) in place of (N/2) log N.
+    Returns (l, variance1, variance2)."""
+    N = len(v)
+    dotp = float(v @ v)
+    half = N // 2
+    if kernel.startswith("bivariate_matern_parsimonious2"):
+        v1, v2 = v[:half], v[half:]
+    else:
+        v1, v2 = v[0::2], v[1::2]
+    if approx and not kernel.startswith("bivariate_matern_parsimonious2"):
+        l = -(N / 2.0) + (N / 2.0) * math.log(N / 2.0) - (N / 2.0) * math.log(dotp) - 0.5 * logdet \
+            - (N / 2.0) * LOG_2PI
+        return l, float(v1 @ v1) / (N / 2.0), float(v2 @ v2) / (N / 2.0)
+    l = -half + half * math.log(N) - half * math.log(dotp) - 0.5 * logdet - (N / 2.0) * LOG_2PI
+    return l, float(v1 @ v1) / half, float(v2 @ v2) / half
+
+
+def dmle_profile(kernel: str, x, y, z, theta, metric: int = 0):
+    """EXAGEOSTAT_dmle_Tile with a "_profile" kernel (MLE_dexact.c:372-421),
+    dense.  Returns (l, variance1, variance2, logdet, dotp)."""
+    S = sigma(kernel, x, y, theta, metric)
+    L = np.linalg.cholesky(S)
+    d = np.diag(L)
+    logdet = 2.0 * float(np.sum(np.log(d[d > 0])))
+    v = sla.solve_triangular(L, np.asarray(z, dtype=np.float64), lower=True)
+    l, v1, v2 = profile_from_solve(kernel, v, logdet)
+    return l, v1, v2, logdet, float(v @ v)
+
+
+def band_cholesky_tiles(S: np.ndarray, ts: int, band: int) -> np.ndarray:
+    """CHAM_pdpotrf_diag (exageostat_approx/src/compute/pdpotrf_diag.c:
+    64-110) on the lower tiles of S (ts x ts tiles, ragged last tile):
+    for k: potrf(A_kk); A_mk <- A_mk A_kk^-T for k < m < min(mt, k + band);
+    for k < n < min(mt, k + band): A_nn -= A_nk A_nk^T and
+    A_mn -= A_mk A_nk^T for n < m < min(mt, n + band).  Tiles outside
+    |m - n| < band are never written and keep the covariance; A_mk with
+    m >= k + band enters the updates unsolved, as there.  Returns the
+    dense lower-triangular result (upper triangle zero)."""
+    n = S.shape[0]
+    mt = (n + ts - 1) // ts
+    A = np.tril(np.array(S, dtype=np.float64))
+    sl = [slice(i * ts, min(n, (i + 1) * ts)) for i in range(mt)]
+    for k in range(mt):
+        A[sl[k], sl[k]] = np.linalg.cholesky(A[sl[k], sl[k]])
+        Lkk = A[sl[k], sl[k]]
+        for m in range(k + 1, min(mt, k + band)):
+            A[sl[m], sl[k]] = sla.solve_triangular(Lkk, A[sl[m], sl[k]].T, lower=True).T
+        for nn in range(k + 1, min(mt, k + band)):
+            A[sl[nn], sl[nn]] -= np.tril(A[sl[nn], sl[k]] @ A[sl[nn], sl[k]].T)
+            for m in range(nn + 1, min(mt, nn + band)):
+                A[sl[m], sl[nn]] -= A[sl[m], sl[k]] @ A[sl[nn], sl[k]].T
+    return A
+
+
+def dmle_band(kernel: str, x, y, z, theta, ts: int, band: int, metric: int = 0):
+    """EXAGEOSTAT_dmle_diag_Tile (exageostat_approx/src/compute/
+    MLE_approx.c:201-420): band_cholesky_tiles, logdet = 2 sum log L_ii
+    over positive pivots (core_zmdet.c:50-59), v = L^-1 z with the whole
+    lower triangle of the result (CHAMELEON_dtrsm_Tile, MLE_approx.c:365),
+    l = -v.v/2 - logdet/2 - (N/2) log 2 pi (the profile kernels as in
+    profile_from_solve with approx=True).  Returns (l, logdet, dotp) or,
+    for the profile kernels, (l, variance1, variance2, logdet, dotp)."""
+    S = sigma(kernel, x, y, theta, metric)
+    L = band_cholesky_tiles(S, ts, band)
+    d = np.diag(L)
+    logdet = 2.0 * float(np.sum(np.log(d[d > 0])))
+    v = sla.solve_triangular(L, np.asarray(z, dtype=np.float64), lower=True)
+    dotp = float(v @ v)
+    if kernel.endswith("_profile"):
+        l, v1, v2 = profile_from_solve(kernel, v, logdet, approx=True)
+        return l, v1, v2, logdet, dotp
+    n = S.shape[0]
+    return -0.5 * dotp - 0.5 * logdet - (n / 2.0) * LOG_2PI, logdet, dotp
+
+
+def predict(kernel: str, xobs, yobs, zobs, xmiss, ymiss, theta, zactual=None, metric: int = 0,
+            tobs=None, tmiss=None):
     """EXAGEOSTAT_dmle_Predict_Tile (MLE_dexact.c:688-874):
     C22 = Sigma(lobs, lobs), C12 = Sigma(lmiss, lobs), zmiss = C12 C22^-1 zobs
     (dposv + dgemm); mse = sum (zactual - zmiss)^2 / nZmiss
@@ -68,8 +155,8 @@ def predict(kernel: str, xobs, yobs, zobs, xmiss, ymiss, theta, zactual=None, me
     nZmiss counts rows (2 per location) and the per-variable errors are
     divided by nZmiss/2 (codelet_zmse_bivariate.c:22-51).
     Returns (zmiss, mse, mse1, mse2)."""
-    C22 = matern.cov_matrix(kernel, xobs, yobs, xobs, yobs, theta, metric)
-    C12 = matern.cov_matrix(kernel, xmiss, ymiss, xobs, yobs, theta, metric)
+    C22 = matern.cov_matrix(kernel, xobs, yobs, xobs, yobs, theta, metric, t1=tobs, t2=tobs)
+    C12 = matern.cov_matrix(kernel, xmiss, ymiss, xobs, yobs, theta, metric, t1=tmiss, t2=tobs)
     cf = sla.cho_factor(C22, lower=True)
     alpha = sla.cho_solve(cf, np.asarray(zobs, dtype=np.float64))
     zmiss = C12 @ alpha
@@ -78,6 +165,9 @@ def predict(kernel: str, xobs, yobs, zobs, xmiss, ymiss, theta, zactual=None, me
     err = (np.asarray(zactual, dtype=np.float64) - zmiss) ** 2
     nm = len(zmiss)
     mse = float(np.sum(err)) / nm
+    if kernel.startswith("bivariate_matern_parsimonious2"):
+        h = nm // 2
+        return zmiss, mse, float(np.sum(err[:h])) / h, float(np.sum(err[h:])) / h
     if kernel.startswith("bivariate"):
         return zmiss, mse, float(np.sum(err[0::2])) / (nm // 2), float(np.sum(err[1::2])) / (nm // 2)
     return zmiss, mse, None, None

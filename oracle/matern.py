@@ -167,20 +167,77 @@ def bivariate_derivatives(x1, y1, x2, y2, theta, metric: int = 0):
     return out
 
 
+def core_dcmg_bivariate_parsimonious2(x1, y1, x2, y2, theta, metric: int = 0) -> np.ndarray:
+    """The block layout of "bivariate_matern_parsimonious2"
+    (exageostat_exact/src/compute/MLE_dexact.c:323-360): three univariate
+    core_dcmg generations into the quadrants of C.  Rows 0..m1-1 are
+    variable 1 at locations 0..m1-1, rows m1..2m1-1 variable 2 (columns
+    likewise with m2):
+      C11 = core_dcmg(sigma1^2, beta, nu1),
+      C21 = C12 = core_dcmg(rho~ sqrt(sigma1^2 sigma2^2), beta, nu12),
+      C22 = core_dcmg(sigma2^2, beta, nu2)."""
+    s1, s2, beta, nu1, nu2, _ = (float(t) for t in theta[:6])
+    nu12, _, s12 = parsimonious_constants(theta)
+    m1, m2 = len(x1), len(x2)
+    out = np.empty((2 * m1, 2 * m2))
+    out[:m1, :m2] = core_dcmg(x1, y1, x2, y2, (s1, beta, nu1), metric)
+    c12 = core_dcmg(x1, y1, x2, y2, (s12, beta, nu12), metric)
+    out[m1:, :m2] = c12
+    out[:m1, m2:] = c12
+    out[m1:, m2:] = core_dcmg(x1, y1, x2, y2, (s2, beta, nu2), metric)
+    return out
+
+
+def core_dcmg_spacetime_matern(x1, y1, t1, x2, y2, t2, theta, metric: int = 0) -> np.ndarray:
+    """core_dcmg_spacetime_matern (exageostat_exact/cpu_core/compute/
+    core_zcmg.c:1320-1357), Gneiting (2002) eq. 16 as the reference writes
+    it.  theta = (sigma^2, spatial range, nu, time range a, time smoothness
+    alpha, non-separability beta, delta); t is the location's z coordinate
+    (time).  With the spatial distance d (2-D: calculateDistance with
+    z_flag = 1 leaves time out) and u = |t_i - t_j|:
+      psi  = u^(2 alpha) / a + 1,
+      x    = (d / range) / psi^(beta/2),
+      A    = sigma^2 / (2^(nu-1) Gamma(nu)) x^nu K_nu(x) / psi^(beta+delta),
+    and A = sigma^2 / psi^(beta+delta) where d / range == 0."""
+    s2, rng, nu, a, alpha, beta, delta = (float(v) for v in theta[:7])
+    ds = distance_matrix(x1, y1, x2, y2, metric) / rng
+    u = np.abs(np.asarray(t1, dtype=np.float64)[:, None] - np.asarray(t2, dtype=np.float64)[None, :])
+    psi = np.power(u, 2.0 * alpha) / a + 1.0
+    x = ds / np.power(psi, beta / 2.0)
+    corr = matern_correlation(x, nu)
+    corr[ds == 0] = 1.0
+    return s2 * corr / np.power(psi, beta + delta)
+
+
 KERNELS = {
     "univariate_matern_stationary": (3, core_dcmg),
     "univariate_matern_nuggets_stationary": (4, core_dcmg_nuggets),
     "bivariate_matern_parsimonious": (6, core_dcmg_bivariate_parsimonious),
+    "bivariate_matern_parsimonious_profile": (6, core_dcmg_bivariate_parsimonious),
+    "bivariate_matern_parsimonious2": (6, core_dcmg_bivariate_parsimonious2),
+    "bivariate_matern_parsimonious2_profile": (6, core_dcmg_bivariate_parsimonious2),
+    "univariate_spacetime_matern_stationary": (7, None),
 }
 
 
-def cov_matrix(kernel: str, x1, y1, x2, y2, theta, metric: int = 0) -> np.ndarray:
-    """Dispatch on the reference's kernel strings (codelet_zcmg.c:262-310)."""
+def cov_matrix(kernel: str, x1, y1, x2, y2, theta, metric: int = 0, t1=None, t2=None) -> np.ndarray:
+    """Dispatch on the reference's kernel strings (codelet_zcmg.c:262-310;
+    the "_profile" variants generate the same covariance).  t1/t2: the time
+    coordinates of the space-time kernel."""
     try:
         _, fn = KERNELS[kernel]
     except KeyError:
         raise ValueError(f"unknown kernel {kernel!r}") from None
+    if kernel == "univariate_spacetime_matern_stationary":
+        if t1 is None or t2 is None:
+            raise ValueError("the space-time kernel needs time coordinates")
+        return core_dcmg_spacetime_matern(x1, y1, t1, x2, y2, t2, theta, metric)
     return fn(x1, y1, x2, y2, theta, metric)
+
+
+def num_vars(kernel: str) -> int:
+    """Covariance rows per location (2 for the bivariate kernels)."""
+    return 2 if kernel.startswith("bivariate") else 1
 
 
 def num_params(kernel: str) -> int:

@@ -107,8 +107,25 @@ def bivariate_small():
                 logdet=np.array([l[1]]))
 
 
+def c1_csv():
+    """The n = 1,600 locations and observations of c1_seed0.npz as the flat
+    files the reference reads (misc/compute/flat_file.c: "x,y" per line,
+    one observation per line), written with 17 significant digits so that
+    they parse back to the same doubles."""
+    g = np.load(os.path.join(OUT, "c1_seed0.npz"))
+    locs = os.path.join(OUT, "c1_locs.csv")
+    obs = os.path.join(OUT, "c1_obs.csv")
+    with open(locs, "w") as f:
+        for a, b in zip(g["x"], g["y"]):
+            f.write(f"{a:.17g},{b:.17g}\n")
+    with open(obs, "w") as f:
+        for v in g["z"]:
+            f.write(f"{v:.17g}\n")
+    return [locs, obs]
+
+
 def main():
-    paths = [knu_table(), datagen_small(), c1(0), c1(1), nugget_small(), bivariate_small()]
+    paths = [knu_table(), datagen_small(), c1(0), c1(1), nugget_small(), bivariate_small()] + c1_csv()
     manifest = {}
     for p in paths:
         with open(p, "rb") as f:

@@ -146,3 +146,29 @@ def test_library_generate_xy_loc_matches_oracle():
         x, y = exageostat_amd.generate_xy_loc(n, seed)
         xo, yo, _ = datagen.generate_xy_loc(n, seed)
         assert np.array_equal(x, xo) and np.array_equal(y, yo)
+
+
+def test_generate_xy_loc_st_shares_the_grid_stream():
+    """GenerateXYLoc_ST draws the same jittered grid as GenerateXYLoc (same
+    rand() stream, x before y): Morton-sorting its first slot gives
+    GenerateXYLoc's points bit for bit; every slot repeats them with time
+    slot + 1."""
+    for n, seed, slots in [(100, 1, 3), (1600, 0, 2), (50, 5, 4)]:
+        x, y, z, _ = datagen.generate_xy_loc_st(n, slots, seed)
+        xs, ys = datagen.zsort_locations(x[:n], y[:n])
+        xo, yo, _ = datagen.generate_xy_loc(n, seed)
+        assert np.array_equal(xs, xo) and np.array_equal(ys, yo)
+        for t in range(slots):
+            assert np.array_equal(x[t * n:(t + 1) * n], x[:n]) and np.all(z[t * n:(t + 1) * n] == t + 1)
+
+
+def test_library_generate_xy_loc_st_matches_oracle():
+    import exageostat_amd
+    try:
+        exageostat_amd.load()
+    except RuntimeError as exc:
+        pytest.skip(str(exc))
+    for n, slots, seed in [(100, 3, 1), (1000, 2, 0)]:
+        x, y, z = exageostat_amd.generate_xy_loc_st(n, slots, seed)
+        xo, yo, zo, _ = datagen.generate_xy_loc_st(n, slots, seed)
+        assert np.array_equal(x, xo) and np.array_equal(y, yo) and np.array_equal(z, zo)
