@@ -141,6 +141,7 @@ _SIGNATURES = {
     "EXAGEOSTAT_dpotrf_Tile": (ctypes.c_int, [ctypes.c_int, _P(Desc)]),
     "EXAGEOSTAT_dpotri_Tile": (ctypes.c_int, [ctypes.c_int, _P(Desc)]),
     "EXAGEOSTAT_dtrsv_Tile": (ctypes.c_int, [ctypes.c_int, _P(Desc), c_double_p]),
+    "EXAGEOSTAT_dtrsv_Tile_noalloc": (ctypes.c_int, [ctypes.c_int, _P(Desc), c_double_p]),
     "exageostat_dlarnv": (None, [ctypes.c_int, _P(ctypes.c_int), ctypes.c_int, c_double_p]),
     "exageostat_bobyqa": (ctypes.c_int, [ctypes.c_uint, OBJECTIVE, ctypes.c_void_p, c_double_p, c_double_p,
                                          c_double_p, ctypes.c_double, ctypes.c_int, c_double_p,

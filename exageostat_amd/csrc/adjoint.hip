@@ -33,6 +33,7 @@ struct AdjointState {
   DeviceVector zero;
   double* P = nullptr;  // mt tiles: L_jk X_kk of the current column
   double* Q = nullptr;  // mt tiles: split-k terms of the diagonal tile
+  double* W = nullptr;  // kPotriSplit * mt tiles: split-j slabs of the column stage
   long tiles = 0;
   double* partial = nullptr;
   long partial_blocks = 0;
@@ -43,18 +44,21 @@ struct AdjointState {
     if (ko) (void)hipFree(ko);
     if (P) (void)hipFree(P);
     if (Q) (void)hipFree(Q);
+    if (W) (void)hipFree(W);
     if (partial) (void)hipFree(partial);
     if (sums) (void)hipFree(sums);
   }
 };
 
 static constexpr int kGroups = 256;
+static constexpr int kPotriSplit = 4;
 
 void destroy_adjoint(AdjointState* a) { delete a; }
 
 // Z = Sigma^-1 in place of the factor held in A (tile_potrf output, with the
 // inverted diagonal tiles in ws).
-void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, hipStream_t s) {
+void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, hipStream_t s, double* W,
+                        int max_split) {
   if (ws.factored != A.tile_raw(0, 0) || ws.mt_ < A.mt || ws.ts_ != A.ts)
     throw std::invalid_argument("potri: the matrix is not the last one factored");
   const TileView v = A.view();
@@ -67,7 +71,7 @@ void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, 
       profiled("potri_trmm", below * rk * rk, 0.0, s,
                [&] { launch_takahashi_stage(v, k, ws.xinv(k), P, Q, A.tile(k, k), 0, s); });
       profiled("potri_col", 2.0 * below * below * rk, 0.0, s,
-               [&] { launch_takahashi_stage(v, k, ws.xinv(k), P, Q, A.tile(k, k), 1, s); });
+               [&] { launch_takahashi_stage(v, k, ws.xinv(k), P, Q, A.tile(k, k), 1, s, W, max_split); });
     }
     profiled("potri_diag", 2.0 * below * rk * rk + rk * rk * rk / 3.0, 0.0, s,
              [&] { launch_takahashi_stage(v, k, ws.xinv(k), P, Q, A.tile(k, k), 2, s); });
@@ -85,9 +89,11 @@ static AdjointState& adjoint_buffers(Context& ctx) {
   if (st.tiles < A.mt) {
     if (st.P) (void)hipFree(st.P);
     if (st.Q) (void)hipFree(st.Q);
-    st.P = st.Q = nullptr;
+    if (st.W) (void)hipFree(st.W);
+    st.P = st.Q = st.W = nullptr;
     EXA_HIP(hipMalloc(&st.P, sizeof(double) * tile_elems * A.mt));
     EXA_HIP(hipMalloc(&st.Q, sizeof(double) * tile_elems * A.mt));
+    EXA_HIP(hipMalloc(&st.W, sizeof(double) * tile_elems * A.mt * kPotriSplit));
     st.tiles = A.mt;
   }
   const long nb = grad_partial_blocks(A.view());
@@ -152,7 +158,7 @@ double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int 
   EXA_HIP(hipMemcpyAsync(st.alpha.d, ctx.work.d, sizeof(double) * ctx.work.padded, hipMemcpyDeviceToDevice, s));
   profiled("trsv", (double)ctx.N * (ctx.N + 1.0), 4.0 * (double)ctx.N * (ctx.N + 1.0), s,
            [&] { tile_trsv(A, ctx.ws, st.alpha.d, 1, s); });
-  tile_potri_inplace(A, ctx.ws, st.P, st.Q, s);
+  tile_potri_inplace(A, ctx.ws, st.P, st.Q, s, st.W, kPotriSplit);
 
   const int nsum = kid == kBivariate ? 6 : 4;
   if (kid == kBivariate) {
@@ -229,7 +235,7 @@ int fisher_information(Context& ctx, KernelId kid, const double* theta, int metr
     st.y_tiles = (long)A.mt * A.mt;
   }
   st.zero.allocate(ctx.N, ctx.work.padded);
-  tile_potri_inplace(A, ctx.ws, st.P, st.Q, s);
+  tile_potri_inplace(A, ctx.ws, st.P, st.Q, s, st.W, kPotriSplit);
   const MaternGradParams G = matern_grad_params(ctx, kid, theta, metric);
   const double n = (double)ctx.N;
   for (int j = 0; j < np; ++j) {

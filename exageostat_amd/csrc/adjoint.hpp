@@ -11,7 +11,10 @@ namespace exa {
 // The factor is gone afterwards: ws no longer names A as factored, so a
 // later potri or factor-dependent call on A fails instead of reading Sigma^-1
 // as if it were L.
-void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, hipStream_t s);
+// W (may be null): slabs of max_split * (mt - 1) tiles that let
+// the column stage split its sum over j (fills the GPU evenly).
+void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, hipStream_t s, double* W = nullptr,
+                        int max_split = 1);
 
 double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric, double* grad,
                            double* logdet_out, double* dotp_out, int* info);

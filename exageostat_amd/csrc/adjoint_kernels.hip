@@ -46,19 +46,31 @@ __global__ void __launch_bounds__(C::NT) k_trmm_panel(TileView v, int k, const d
   acc_store_t<C>(acc, P + (long)j * ts * ts + bj * C::BN + (long)bi * C::BM * ts, ts);
 }
 
-// Z_mk = - sum_{j>k} Z(m,j) P_j for m in (k, mt).
+// Z_mk = - sum_{j>k} Z(m,j) P_j for m in (k, mt).  With nsplit > 1 the
+// sum over j is cut into nsplit contiguous ranges; work-group set s writes
+// its partial sum of range s to slab s of W (below tiles of ts x ts each,
+// tile m - k - 1 at W + (s * below + m - k - 1) ts^2) and k_sum_slabs adds
+// the slabs into Z_mk in a fixed order.  More, smaller work units fill the
+// GPU's CUs evenly when (mt - k - 1) * (ts / BM)^2 blocks alone would leave
+// a partial last round.
 template <class C>
-__global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, const double* P) {
+__global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, const double* P, double* W, int nsplit) {
   __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
   const int qm = v.ts / C::BM, qn = v.ts / C::BN;
-  const long b = blockIdx.x;
+  const long below = v.mt - k - 1;
+  const long per_split = below * qm * qn;
+  const int sp = (int)(blockIdx.x / per_split);
+  const long b = blockIdx.x - (long)sp * per_split;
   const int m = k + 1 + (int)(b / (qm * qn));
   const int rem = (int)(b % (qm * qn));
   const int bi = rem % qm, bj = rem / qm;
   const long ts = v.ts;
+  const int chunk = (int)((below + nsplit - 1) / nsplit);
+  const int j_lo = k + 1 + sp * chunk;
+  const int j_hi = j_lo + chunk < v.mt ? j_lo + chunk : v.mt;
   dx4 acc[C::FM][C::FN];
   zero_acc<C>(acc);
-  for (int j = k + 1; j < v.mt; ++j) {
+  for (int j = j_lo; j < j_hi; ++j) {
     const double* B = P + (long)j * ts * ts + bj * C::BN;  // B(c, kk) = P_j^T[bj*BN + c, kk]
     if (j <= m) {
       const double* A = tile_f64(v, m, j) + bi * C::BM;
@@ -68,7 +80,20 @@ __global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, cons
       gemm_nt_acc<C, false, double, double, true>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
     }
   }
-  acc_update<C>(acc, tile_f64(v, m, k) + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
+  double* out = nsplit > 1 ? W + ((long)sp * below + (m - k - 1)) * ts * ts : tile_f64(v, m, k);
+  acc_update<C>(acc, out + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
+}
+
+// Z_mk = sum_s W[s][m - k - 1] for m in (k, mt), elementwise, s ascending.
+__global__ void k_sum_slabs(TileView v, int k, const double* W, int nsplit) {
+  const long len = (long)v.ts * v.ts;
+  const long below = v.mt - k - 1;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= below * len) return;
+  const long t = e / len;
+  double acc = 0.0;
+  for (int sp = 0; sp < nsplit; ++sp) acc += W[((long)sp * below) * len + e];
+  tile_f64(v, k + 1 + (int)t, k)[e - t * len] = acc;
 }
 
 // Split-k terms of Z_kk: Q_0 = X_kk^T X_kk, Q_i = -Z_{k+i,k}^T P_{k+i}.
@@ -106,16 +131,54 @@ __global__ void k_sum_tiles(const double* Q, int cnt, long len, double* out) {
   out[e] = s;
 }
 
+// Number of j ranges of the column stage: the split (at most 4, and at most
+// one tile per range, and only as many as the slab buffer holds) whose work
+// units fill whole rounds of the concurrent work-groups best; each extra
+// range costs a write and a read of its slab, counted as 2% per range.
 template <class C>
-static void takahashi_cfg(const TileView& v, int k, const double* X, double* P, double* Q, int stage,
-                          hipStream_t s) {
+static int takahashi_splits(const TileView& v, int k, int max_split) {
+  static int slots = 0;
+  if (!slots) {
+    int dev = 0, cus = 0, occ = 1;
+    EXA_HIP(hipGetDevice(&dev));
+    EXA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    EXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_takahashi_col<C>, C::NT, 0));
+    slots = cus * (occ > 0 ? occ : 1);
+  }
+  const long per = (long)(v.ts / C::BM) * (v.ts / C::BN);
+  const long below = v.mt - k - 1;
+  int best = 1;
+  double best_cost = 1e300;
+  for (int sp = 1; sp <= 4 && sp <= below && sp <= max_split; ++sp) {
+    const long units = below * per * sp;
+    const double rounds = (double)((units + slots - 1) / slots);
+    // time in units of one unsplit block's work
+    const double cost = rounds / sp * (1.0 + 0.02 * (sp - 1));
+    if (cost < best_cost - 1e-12) {
+      best_cost = cost;
+      best = sp;
+    }
+  }
+  return best;
+}
+
+template <class C>
+static void takahashi_cfg(const TileView& v, int k, const double* X, double* P, double* Q, double* W, int max_split,
+                          int stage, hipStream_t s) {
   const long per = (long)(v.ts / C::BM) * (v.ts / C::BN);
   const long below = v.mt - k - 1;
   if (stage == 0 && below > 0)
     hipLaunchKernelGGL(k_trmm_panel<C>, dim3((unsigned)(below * per)), dim3(C::NT), 0, s, v, k, X, P);
-  if (stage == 1 && below > 0)
-    hipLaunchKernelGGL(k_takahashi_col<C>, dim3((unsigned)(below * per)), dim3(C::NT), 0, s, v, k,
-                       (const double*)P);
+  if (stage == 1 && below > 0) {
+    const int ns = W ? takahashi_splits<C>(v, k, max_split) : 1;
+    hipLaunchKernelGGL(k_takahashi_col<C>, dim3((unsigned)(below * per * ns)), dim3(C::NT), 0, s, v, k,
+                       (const double*)P, W, ns);
+    if (ns > 1) {
+      const long len = below * (long)v.ts * v.ts;
+      hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, v, k, (const double*)W,
+                         ns);
+    }
+  }
   if (stage == 2)
     hipLaunchKernelGGL(k_takahashi_diag<C>, dim3((unsigned)((below + 1) * per)), dim3(C::NT), 0, s, v, k, X,
                        (const double*)P, Q);
@@ -123,10 +186,10 @@ static void takahashi_cfg(const TileView& v, int k, const double* X, double* P, 
 }
 
 void launch_takahashi_stage(const TileView& v, int k, const double* X, double* P, double* Q, double* Zkk,
-                            int stage, hipStream_t s) {
-  if (v.ts % 192 == 0) takahashi_cfg<C192>(v, k, X, P, Q, stage, s);
-  else if (v.ts % 128 == 0) takahashi_cfg<C128>(v, k, X, P, Q, stage, s);
-  else takahashi_cfg<CT64>(v, k, X, P, Q, stage, s);
+                            int stage, hipStream_t s, double* W, int max_split) {
+  if (v.ts % 192 == 0) takahashi_cfg<C192>(v, k, X, P, Q, W, max_split, stage, s);
+  else if (v.ts % 128 == 0) takahashi_cfg<C128>(v, k, X, P, Q, W, max_split, stage, s);
+  else takahashi_cfg<CT64>(v, k, X, P, Q, W, max_split, stage, s);
   if (stage == 2) {
     const long len = (long)v.ts * v.ts;
     hipLaunchKernelGGL(k_sum_tiles, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, (const double*)Q,

@@ -150,6 +150,37 @@ hipStream_t seq_stream(RUNTIME_sequence_t* seq) {
   return rt().s1;
 }
 
+// Scratch of EXAGEOSTAT_dtrsv_Tile_noalloc, the entry point that allocates
+// nothing: the padded right-hand side (kNoAllocRows doubles), the solve's
+// assembly vector (kNoAllocRows), the partial sums of the diagonal-tile
+// products ((kNoAllocTs / 64) * kNoAllocTs) and the completion ticket.
+// 1.13 MB per GPU, held until the process ends.  Host code never passes
+// `scratch` itself (its host-side name is not a device address): it uses
+// the device address that hipGetSymbolAddress returns for the current
+// device.
+constexpr long kNoAllocRows = 65536;  // mt * ts at most
+constexpr long kNoAllocTs = 1024;
+constexpr long kNoAllocPart = (kNoAllocTs / kNB) * kNoAllocTs;
+constexpr long kNoAllocDoubles = 2 * kNoAllocRows + kNoAllocPart + 32;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+// Device address of `scratch` on the current device, looked up once per
+// device.
+double* scratch_ptr() {
+  static std::vector<double*> ptrs;
+  int dev = 0;
+  EXA_HIP(hipGetDevice(&dev));
+  if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+    ptrs[dev] = static_cast<double*>(p);
+  }
+  return ptrs[dev];
+}
+
 Context* ctx_of(MLE_data* data) {
   if (!data || !data->gpu) throw std::runtime_error("MLE_data has no GPU context (call EXAGEOSTAT_dmle_Call)");
   return static_cast<Context*>(data->gpu);
@@ -708,6 +739,36 @@ int EXAGEOSTAT_dtrsv_Tile(CHAM_enum trans, CHAM_desc_t* L, double* b) {
     tile_trsv(tm, rt().ws, v.d, trans == ChamTrans ? 1 : 0, rt().s1);
     EXA_HIP(hipMemcpyAsync(b, v.d, sizeof(double) * tm.m, hipMemcpyDeviceToHost, rt().s1));
     EXA_HIP(hipStreamSynchronize(rt().s1));
+    return 0;
+  }, -1);
+}
+
+int EXAGEOSTAT_dtrsv_Tile_noalloc(CHAM_enum trans, CHAM_desc_t* L, double* b) {
+  return guarded([&]() {
+    auto* impl = static_cast<DescImpl*>(L->mat);
+    if (!impl || !impl->tm) throw std::invalid_argument("L must be a tile-matrix descriptor");
+    TileMatrix& tm = *impl->tm;
+    const long padded = (long)tm.mt * tm.ts;
+    if (padded > kNoAllocRows || tm.ts > kNoAllocTs)
+      throw std::invalid_argument("EXAGEOSTAT_dtrsv_Tile_noalloc: mt * ts = " + std::to_string(padded) +
+                                  " and ts = " + std::to_string(tm.ts) + " must not exceed " +
+                                  std::to_string(kNoAllocRows) + " and " + std::to_string(kNoAllocTs));
+    if (rt().ws.factored != tm.tile_raw(0, 0) || rt().ws.mt_ < tm.mt || rt().ws.ts_ != tm.ts)
+      throw std::invalid_argument("EXAGEOSTAT_dtrsv_Tile_noalloc: L is not the last matrix factored by "
+                                  "EXAGEOSTAT_dpotrf_Tile");
+    double* base = scratch_ptr();
+    double* v = base;
+    double* y = base + kNoAllocRows;
+    double* tpart = y + kNoAllocRows;
+    unsigned* ticket = reinterpret_cast<unsigned*>(tpart + kNoAllocPart);
+    hipStream_t s = rt().s1;
+    // the padding rows of v must be zero; the ticket starts at zero
+    EXA_HIP(hipMemsetAsync(v, 0, sizeof(double) * padded, s));
+    EXA_HIP(hipMemsetAsync(ticket, 0, sizeof(unsigned), s));
+    EXA_HIP(hipMemcpyAsync(v, b, sizeof(double) * tm.m, hipMemcpyHostToDevice, s));
+    tile_trsv_scratch(tm, rt().ws, v, trans == ChamTrans ? 1 : 0, s, y, tpart, ticket);
+    EXA_HIP(hipMemcpyAsync(b, v, sizeof(double) * tm.m, hipMemcpyDeviceToHost, s));
+    EXA_HIP(hipStreamSynchronize(s));
     return 0;
   }, -1);
 }

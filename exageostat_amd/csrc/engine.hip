@@ -601,20 +601,24 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s1, hipStream_t s2,
 }
 
 void tile_trsv(const TileMatrix& L, const CholWorkspace& ws, double* z, int trans, hipStream_t s) {
+  tile_trsv_scratch(L, ws, z, trans, s, ws.yvec(), ws.tpart(), ws.ticket());
+}
+
+void tile_trsv_scratch(const TileMatrix& L, const CholWorkspace& ws, double* z, int trans, hipStream_t s, double* y,
+                       double* tpart, unsigned* ticket) {
   // Blocked substitution with the inverted diagonal tiles: y_k = X_k z_k
   // (resp. X_k^T z_k), then the off-diagonal tiles update the remaining
   // right-hand side; y is assembled in a scratch vector and copied back.
   TileView v = L.view();
   const int ts = L.ts;
-  double* y = ws.yvec();
   if (!trans) {
     for (int k = 0; k < L.mt; ++k) {
-      launch_trmv_diag(ws.xinv(k), ts, z + (long)k * ts, y + (long)k * ts, 0, ws.tpart(), ws.ticket(), s);
+      launch_trmv_diag(ws.xinv(k), ts, z + (long)k * ts, y + (long)k * ts, 0, tpart, ticket, s);
       launch_gemv_panel(v, k, y + (long)k * ts, z, s);
     }
   } else {
     for (int k = L.mt - 1; k >= 0; --k) {
-      launch_trmv_diag(ws.xinv(k), ts, z + (long)k * ts, y + (long)k * ts, 1, ws.tpart(), ws.ticket(), s);
+      launch_trmv_diag(ws.xinv(k), ts, z + (long)k * ts, y + (long)k * ts, 1, tpart, ticket, s);
       launch_gemv_t_row(v, k, y + (long)k * ts, z, s);
     }
   }

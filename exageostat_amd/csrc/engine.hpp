@@ -176,6 +176,10 @@ int tile_potrf(TileMatrix& A, CholWorkspace& ws, hipStream_t s_main, hipStream_t
                const ClassLists* lists = nullptr, int band = 0);
 // z <- L^-1 z (trans=0) or L^-T z (trans=1) with the factor from tile_potrf.
 void tile_trsv(const TileMatrix& L, const CholWorkspace& ws, double* z, int trans, hipStream_t s);
+// The same with caller-provided scratch: y (mt*ts doubles), tpart
+// ((ts/64)*ts doubles) and ticket (zero on entry; left zero).
+void tile_trsv_scratch(const TileMatrix& L, const CholWorkspace& ws, double* z, int trans, hipStream_t s, double* y,
+                       double* tpart, unsigned* ticket);
 // sum_i log L_ii (deterministic order).
 double tile_logdet(const TileMatrix& L, double* d_partial, hipStream_t s);
 double device_dot(const double* a, const double* b, long n, double* d_partial, hipStream_t s);

@@ -144,8 +144,10 @@ struct MaternGradParams {
 // One stage of tile column k of the Takahashi sweep Z = Sigma^-1 over the
 // factor in v (0: P_j = L_jk X_kk; 1: Z_mk for m > k; 2: Z_kk into Zkk,
 // with Q as split-k scratch of mt tiles).  X = L_kk^-1, P: mt tiles.
+// W (may be null): slabs for splitting stage 1's sum over j into up to
+// max_split ranges, max_split * (mt - 1) tiles.
 void launch_takahashi_stage(const TileView& v, int k, const double* X, double* P, double* Q, double* Zkk,
-                            int stage, hipStream_t s);
+                            int stage, hipStream_t s, double* W = nullptr, int max_split = 1);
 struct BivariateGradParams {
   const double* x;
   const double* y;

@@ -493,6 +493,17 @@ int EXAGEOSTAT_dpotri_Tile(CHAM_enum uplo, CHAM_desc_t *A);
 /* B <- L^-1 B (trans = ChamNoTrans) or L^-T B (ChamTrans), B an n-vector
  * given in host memory; L from EXAGEOSTAT_dpotrf_Tile. */
 int EXAGEOSTAT_dtrsv_Tile(CHAM_enum trans, CHAM_desc_t *L, double *b);
+/* No counterpart in the reference.  EXAGEOSTAT_dtrsv_Tile without any
+ * device allocation, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * its scratch (the padded right-hand side, the solve's assembly vector,
+ * partial sums, a completion counter) is a static device array of 1.13 MB
+ * per GPU, held until the process ends.  Limits: mt * ts <= 65536 and
+ * ts <= 1024; larger L returns -1 with an error string.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (other host threads)
+ * race.  L must come from EXAGEOSTAT_dpotrf_Tile (its inverted diagonal
+ * tiles are read).  Returns 0 or -1. */
+int EXAGEOSTAT_dtrsv_Tile_noalloc(CHAM_enum trans, CHAM_desc_t *L, double *b);
 
 /* LAPACK dlarnv (the reference draws N(0,1) with LAPACKE_dlarnv(3, iseed, N,
  * Nrand), examples/synthetic_dmle_test.c:111,147); iseed is advanced. */
