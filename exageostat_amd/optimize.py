@@ -207,7 +207,9 @@ def main(argv=None) -> None:
     ap.add_argument("--opt-tol", type=int, default=5)
     ap.add_argument("--max-evals", type=int, default=500)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--methods", default="lbfgsb,dfo")
+    ap.add_argument("--methods", default="lbfgsb,bobyqa")
+    ap.add_argument("--bobyqa-start", default="lb", choices=["lb", "start"],
+                    help="BOBYQA's start: the lower bounds (the reference's, examples/examples.c:263-264) or --start")
     args = ap.parse_args(argv)
 
     x, y = mle.generate_xy_loc(args.n, args.seed)
@@ -238,17 +240,19 @@ def main(argv=None) -> None:
     print(f"gradient check {out['grad_check']['max_rel']:.2e}", file=sys.stderr, flush=True)
     res = {}
     for meth in args.methods.split(","):
-        r = fit(m, lb, ub, start=start, method=meth, opt_tol=args.opt_tol, max_evals=args.max_evals)
+        st = None if (meth == "bobyqa" and args.bobyqa_start == "lb") else start
+        r = fit(m, lb, ub, start=st, method=meth, opt_tol=args.opt_tol, max_evals=args.max_evals)
         res[meth] = {"theta": list(r.theta), "loglik": r.loglik, "evals": r.evals, "grad_evals": r.grad_evals,
                      "wall_s": r.wall_s, "message": r.message}
         print(f"{meth}: {r.wall_s:.1f}s {r.evals} evals theta={np.round(r.theta, 6)} l={r.loglik:.6f}",
               file=sys.stderr, flush=True)
     out["fits"] = res
-    if "lbfgsb" in res and "dfo" in res:
-        a, b = np.array(res["lbfgsb"]["theta"]), np.array(res["dfo"]["theta"])
-        out["theta_max_rel_diff"] = float(np.max(np.abs(a - b) / np.abs(b)))
-        out["wall_ratio_dfo_over_lbfgsb"] = res["dfo"]["wall_s"] / res["lbfgsb"]["wall_s"]
-        out["loglik_diff"] = res["lbfgsb"]["loglik"] - res["dfo"]["loglik"]
+    for other in ("bobyqa", "dfo"):
+        if "lbfgsb" in res and other in res:
+            a, b = np.array(res["lbfgsb"]["theta"]), np.array(res[other]["theta"])
+            out[f"theta_max_rel_diff_{other}"] = float(np.max(np.abs(a - b) / np.abs(b)))
+            out[f"wall_ratio_{other}_over_lbfgsb"] = res[other]["wall_s"] / res["lbfgsb"]["wall_s"]
+            out[f"loglik_diff_{other}"] = res["lbfgsb"]["loglik"] - res[other]["loglik"]
     m.close()
     print(json.dumps(out), flush=True)
 
