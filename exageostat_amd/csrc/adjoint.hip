@@ -34,6 +34,7 @@ struct AdjointState {
   double* P = nullptr;  // mt tiles: L_jk X_kk of the current column
   double* Q = nullptr;  // mt tiles: split-k terms of the diagonal tile
   double* W = nullptr;  // kPotriSplit * mt tiles: split-j slabs of the column stage
+  TileMatrix U;         // Z_mk^T of the sweep (lower tiles), read in place of transposed Z tiles
   long tiles = 0;
   double* partial = nullptr;
   long partial_blocks = 0;
@@ -58,7 +59,14 @@ void destroy_adjoint(AdjointState* a) { delete a; }
 // Z = Sigma^-1 in place of the factor held in A (tile_potrf output, with the
 // inverted diagonal tiles in ws).
 void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, hipStream_t s, double* W,
-                        int max_split) {
+                        int max_split, TileMatrix* U) {
+  TileView uv{};
+  if (U) {
+    if (!U->allocated() || U->mt != A.mt || U->ts != A.ts || !U->lower)
+      throw std::invalid_argument("potri: the transposed copy must have the factor's tile grid");
+    uv = U->view();
+  }
+  const TileView* up = U ? &uv : nullptr;
   if (ws.factored != A.tile_raw(0, 0) || ws.mt_ < A.mt || ws.ts_ != A.ts)
     throw std::invalid_argument("potri: the matrix is not the last one factored");
   const TileView v = A.view();
@@ -71,10 +79,10 @@ void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, 
       profiled("potri_trmm", below * rk * rk, 0.0, s,
                [&] { launch_takahashi_stage(v, k, ws.xinv(k), P, Q, A.tile(k, k), 0, s); });
       profiled("potri_col", 2.0 * below * below * rk, 0.0, s,
-               [&] { launch_takahashi_stage(v, k, ws.xinv(k), P, Q, A.tile(k, k), 1, s, W, max_split); });
+               [&] { launch_takahashi_stage(v, k, ws.xinv(k), P, Q, A.tile(k, k), 1, s, W, max_split, up); });
     }
     profiled("potri_diag", 2.0 * below * rk * rk + rk * rk * rk / 3.0, 0.0, s,
-             [&] { launch_takahashi_stage(v, k, ws.xinv(k), P, Q, A.tile(k, k), 2, s); });
+             [&] { launch_takahashi_stage(v, k, ws.xinv(k), P, Q, A.tile(k, k), 2, s, nullptr, 1, up); });
   }
   ws.factored = nullptr;
 }
@@ -104,6 +112,10 @@ static AdjointState& adjoint_buffers(Context& ctx) {
     st.partial_blocks = nb;
   }
   if (!st.sums) EXA_HIP(hipMalloc(&st.sums, sizeof(double) * 8 * kGroups));
+  if (!st.U.allocated() || st.U.m != A.m || st.U.ts != A.ts) {
+    st.U.release();
+    st.U.allocate(A.m, A.m, A.ts, true);
+  }
   return st;
 }
 
@@ -158,7 +170,7 @@ double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int 
   EXA_HIP(hipMemcpyAsync(st.alpha.d, ctx.work.d, sizeof(double) * ctx.work.padded, hipMemcpyDeviceToDevice, s));
   profiled("trsv", (double)ctx.N * (ctx.N + 1.0), 4.0 * (double)ctx.N * (ctx.N + 1.0), s,
            [&] { tile_trsv(A, ctx.ws, st.alpha.d, 1, s); });
-  tile_potri_inplace(A, ctx.ws, st.P, st.Q, s, st.W, kPotriSplit);
+  tile_potri_inplace(A, ctx.ws, st.P, st.Q, s, st.W, kPotriSplit, &st.U);
 
   const int nsum = kid == kBivariate ? 6 : 4;
   if (kid == kBivariate) {
@@ -235,7 +247,7 @@ int fisher_information(Context& ctx, KernelId kid, const double* theta, int metr
     st.y_tiles = (long)A.mt * A.mt;
   }
   st.zero.allocate(ctx.N, ctx.work.padded);
-  tile_potri_inplace(A, ctx.ws, st.P, st.Q, s, st.W, kPotriSplit);
+  tile_potri_inplace(A, ctx.ws, st.P, st.Q, s, st.W, kPotriSplit, &st.U);
   const MaternGradParams G = matern_grad_params(ctx, kid, theta, metric);
   const double n = (double)ctx.N;
   for (int j = 0; j < np; ++j) {

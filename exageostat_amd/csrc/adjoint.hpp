@@ -13,8 +13,11 @@ namespace exa {
 // as if it were L.
 // W (may be null): slabs of max_split * (mt - 1) tiles that let
 // the column stage split its sum over j (fills the GPU evenly).
+// U (may be null): a lower tile matrix of A's grid that receives Z_mk^T, so
+// that the sweep reads no transposed operands (the memory of a second
+// lower-triangular matrix for a faster sweep).
 void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, hipStream_t s, double* W = nullptr,
-                        int max_split = 1);
+                        int max_split = 1, TileMatrix* U = nullptr);
 
 double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric, double* grad,
                            double* logdet_out, double* dotp_out, int* info);

@@ -53,8 +53,12 @@ __global__ void __launch_bounds__(C::NT) k_trmm_panel(TileView v, int k, const d
 // the slabs into Z_mk in a fixed order.  More, smaller work units fill the
 // GPU's CUs evenly when (mt - k - 1) * (ts / BM)^2 blocks alone would leave
 // a partial last round.
+// U (optional, U.ptr null when absent): tile (j, m), j > m, holds Z_jm^T,
+// written by the epilogues of the column stages before it is read, so that
+// every operand is read in its plain (row-contiguous) layout.
 template <class C>
-__global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, const double* P, double* W, int nsplit) {
+__global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, const double* P, double* W, int nsplit,
+                                                         TileView U) {
   __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
   const int qm = v.ts / C::BM, qn = v.ts / C::BN;
   const long below = v.mt - k - 1;
@@ -75,31 +79,56 @@ __global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, cons
     if (j <= m) {
       const double* A = tile_f64(v, m, j) + bi * C::BM;
       gemm_nt_acc<C>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
+    } else if (U.ptr) {
+      const double* A = tile_f64(U, j, m) + bi * C::BM;  // A(r, kk) = Z_jm^T[bi*BM + r, kk]
+      gemm_nt_acc<C>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
     } else {
       const double* A = tile_f64(v, j, m) + (long)bi * C::BM * ts;  // A(r, kk) = Z_jm[kk, bi*BM + r]
       gemm_nt_acc<C, false, double, double, true>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
     }
   }
-  double* out = nsplit > 1 ? W + ((long)sp * below + (m - k - 1)) * ts * ts : tile_f64(v, m, k);
-  acc_update<C>(acc, out + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
+  if (nsplit > 1) {
+    acc_update<C>(acc, W + ((long)sp * below + (m - k - 1)) * ts * ts + bi * C::BM + (long)bj * C::BN * ts, ts,
+                  0.0);
+    return;
+  }
+  if (U.ptr) acc_store_t<C>(acc, tile_f64(U, m, k) + bj * C::BN + (long)bi * C::BM * ts, ts);
+  acc_update<C>(acc, tile_f64(v, m, k) + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
 }
 
-// Z_mk = sum_s W[s][m - k - 1] for m in (k, mt), elementwise, s ascending.
-__global__ void k_sum_slabs(TileView v, int k, const double* W, int nsplit) {
-  const long len = (long)v.ts * v.ts;
+// Z_mk = sum_s W[s][m - k - 1] for m in (k, mt), s ascending, and (with
+// U) Z_mk^T into U's tile (m, k).  One work-group per 64 x 64 block: the sum
+// is written column by column and the transpose goes through LDS so that
+// both stores are contiguous.
+__global__ void __launch_bounds__(256) k_sum_slabs(TileView v, int k, const double* W, int nsplit, TileView U) {
+  __shared__ double blk[64][65];
+  const int ts = v.ts;
+  const int q = ts / 64;
+  const long len = (long)ts * ts;
   const long below = v.mt - k - 1;
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= below * len) return;
-  const long t = e / len;
-  double acc = 0.0;
-  for (int sp = 0; sp < nsplit; ++sp) acc += W[((long)sp * below) * len + e];
-  tile_f64(v, k + 1 + (int)t, k)[e - t * len] = acc;
+  const long b = blockIdx.x;
+  const long t = b / (q * q);
+  const int rem = (int)(b - t * q * q);
+  const int r0 = (rem % q) * 64, c0 = (rem / q) * 64;
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  double* Z = tile_f64(v, k + 1 + (int)t, k);
+  for (int c = grp; c < 64; c += 4) {
+    const long e = (long)(r0 + lane) + (long)(c0 + c) * ts;
+    double acc = 0.0;
+    for (int sp = 0; sp < nsplit; ++sp) acc += W[((long)sp * below + t) * len + e];
+    Z[e] = acc;
+    blk[lane][c] = acc;
+  }
+  if (!U.ptr) return;
+  __syncthreads();
+  double* Ut = tile_f64(U, k + 1 + (int)t, k);
+  for (int r = grp; r < 64; r += 4) Ut[(long)(c0 + lane) + (long)(r0 + r) * ts] = blk[r][lane];
 }
 
 // Split-k terms of Z_kk: Q_0 = X_kk^T X_kk, Q_i = -Z_{k+i,k}^T P_{k+i}.
 template <class C>
 __global__ void __launch_bounds__(C::NT) k_takahashi_diag(TileView v, int k, const double* X, const double* P,
-                                                          double* Q) {
+                                                          double* Q, TileView U) {
   __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
   const int qm = v.ts / C::BM, qn = v.ts / C::BN;
   const long b = blockIdx.x;
@@ -114,6 +143,10 @@ __global__ void __launch_bounds__(C::NT) k_takahashi_diag(TileView v, int k, con
     const int k0 = (bi * C::BM > bj * C::BN ? bi * C::BM : bj * C::BN);
     gemm_nt_acc<C, true, double, double, true>(acc, X + k0 + (long)bi * C::BM * ts, ts,
                                                X + k0 + (long)bj * C::BN * ts, ts, (int)(ts - k0), lds, 1.0);
+  } else if (U.ptr) {
+    const int j = k + i;
+    gemm_nt_acc<C>(acc, tile_f64(U, j, k) + bi * C::BM, ts, P + (long)j * ts * ts + bj * C::BN, ts, (int)ts, lds,
+                   -1.0);
   } else {
     const int j = k + i;
     gemm_nt_acc<C, false, double, double, true>(acc, tile_f64(v, j, k) + (long)bi * C::BM * ts, ts,
@@ -164,7 +197,7 @@ static int takahashi_splits(const TileView& v, int k, int max_split) {
 
 template <class C>
 static void takahashi_cfg(const TileView& v, int k, const double* X, double* P, double* Q, double* W, int max_split,
-                          int stage, hipStream_t s) {
+                          const TileView& U, int stage, hipStream_t s) {
   const long per = (long)(v.ts / C::BM) * (v.ts / C::BN);
   const long below = v.mt - k - 1;
   if (stage == 0 && below > 0)
@@ -172,24 +205,26 @@ static void takahashi_cfg(const TileView& v, int k, const double* X, double* P, 
   if (stage == 1 && below > 0) {
     const int ns = W ? takahashi_splits<C>(v, k, max_split) : 1;
     hipLaunchKernelGGL(k_takahashi_col<C>, dim3((unsigned)(below * per * ns)), dim3(C::NT), 0, s, v, k,
-                       (const double*)P, W, ns);
+                       (const double*)P, W, ns, U);
     if (ns > 1) {
-      const long len = below * (long)v.ts * v.ts;
-      hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, v, k, (const double*)W,
-                         ns);
+      const long q = v.ts / 64;
+      hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)(below * q * q)), dim3(256), 0, s, v, k, (const double*)W, ns,
+                         U);
     }
   }
   if (stage == 2)
     hipLaunchKernelGGL(k_takahashi_diag<C>, dim3((unsigned)((below + 1) * per)), dim3(C::NT), 0, s, v, k, X,
-                       (const double*)P, Q);
+                       (const double*)P, Q, U);
   EXA_HIP(hipGetLastError());
 }
 
 void launch_takahashi_stage(const TileView& v, int k, const double* X, double* P, double* Q, double* Zkk,
-                            int stage, hipStream_t s, double* W, int max_split) {
-  if (v.ts % 192 == 0) takahashi_cfg<C192>(v, k, X, P, Q, W, max_split, stage, s);
-  else if (v.ts % 128 == 0) takahashi_cfg<C128>(v, k, X, P, Q, W, max_split, stage, s);
-  else takahashi_cfg<CT64>(v, k, X, P, Q, W, max_split, stage, s);
+                            int stage, hipStream_t s, double* W, int max_split, const TileView* U) {
+  TileView u{};
+  if (U) u = *U;
+  if (v.ts % 192 == 0) takahashi_cfg<C192>(v, k, X, P, Q, W, max_split, u, stage, s);
+  else if (v.ts % 128 == 0) takahashi_cfg<C128>(v, k, X, P, Q, W, max_split, u, stage, s);
+  else takahashi_cfg<CT64>(v, k, X, P, Q, W, max_split, u, stage, s);
   if (stage == 2) {
     const long len = (long)v.ts * v.ts;
     hipLaunchKernelGGL(k_sum_tiles, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, (const double*)Q,

@@ -145,9 +145,12 @@ struct MaternGradParams {
 // factor in v (0: P_j = L_jk X_kk; 1: Z_mk for m > k; 2: Z_kk into Zkk,
 // with Q as split-k scratch of mt tiles).  X = L_kk^-1, P: mt tiles.
 // W (may be null): slabs for splitting stage 1's sum over j into up to
-// max_split ranges, max_split * (mt - 1) tiles.
+// max_split ranges, max_split * (mt - 1) tiles.  U (may be null): a lower
+// tile matrix of v's geometry that receives Z_mk^T in tile (m, k), so that
+// the later stages read no transposed operands.
 void launch_takahashi_stage(const TileView& v, int k, const double* X, double* P, double* Q, double* Zkk,
-                            int stage, hipStream_t s, double* W = nullptr, int max_split = 1);
+                            int stage, hipStream_t s, double* W = nullptr, int max_split = 1,
+                            const TileView* U = nullptr);
 struct BivariateGradParams {
   const double* x;
   const double* y;

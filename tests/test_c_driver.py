@@ -49,10 +49,14 @@ def test_c_driver_runs_the_reference_sequence(tmp_path):
     lines = r.stdout.strip().splitlines()
     ll = [line.split() for line in lines if line.startswith("loglik")]
     assert len(ll) == len(g["thetas"])
-    for (_, l, logdet, dotp, status), ref, rld, rdp in zip(ll, g["loglik"], g["logdet"], g["dotp"]):
+    # the last golden theta has cond(Sigma) = 2.2e10: one ulp of Sigma's
+    # entries moves l by 1.8e-9 relative in the oracle itself (as in
+    # test_gpu_parity.test_loglik_c1_golden)
+    tols = [1e-9, 1e-9, 1e-9, 1e-9, 1e-7]
+    for (_, l, logdet, dotp, status), ref, rld, rdp, tol in zip(ll, g["loglik"], g["logdet"], g["dotp"], tols):
         assert int(status) == 0
-        assert abs(float(l) - ref) <= 1e-9 * abs(ref), (l, ref)
-        assert abs(float(logdet) - rld) <= 1e-9 * abs(ref) and abs(float(dotp) - rdp) <= 1e-9 * rdp
+        assert abs(float(l) - ref) <= tol * abs(ref), (l, ref)
+        assert abs(float(logdet) - rld) <= tol * abs(ref) and abs(float(dotp) - rdp) <= tol * rdp
     zmiss = np.array([float(v) for v in next(x for x in lines if x.startswith("zmiss")).split()[1:]])
     assert np.max(np.abs(zmiss - g["zmiss"])) <= 1e-9 * np.max(np.abs(g["zmiss"]))
     mse, mse_results = (float(v) for v in next(x for x in lines if x.startswith("mse")).split()[1:])

@@ -56,9 +56,9 @@ def test_noalloc_solve_matches_allocating_solve_twice(lib):
 def test_noalloc_solve_refuses_oversized_input(lib):
     from exageostat_amd import _lib
     c = _lib.load()
-    n, ts = 4096, 2048  # ts above the 1,024 limit
+    n, ts = 65537, 1024  # mt * ts = 66,560 > 65,536 (18 GB of lower tiles, allocated, never computed on)
     M = lib.DeviceMatrix(n, n, ts, lower=True)
     b = np.zeros(n)
     rc = c.EXAGEOSTAT_dtrsv_Tile_noalloc(_lib.ChamNoTrans, M.desc, b.ctypes.data_as(_lib.c_double_p))
-    assert rc == -1 and b"1024" in c.exageostat_gpu_last_error()
+    assert rc == -1 and b"65536" in c.exageostat_gpu_last_error()
     M.close()
