@@ -32,12 +32,6 @@ enum TilePrec : uint8_t { kF64 = 0, kF32 = 1, kBF16 = 2, kF32B = 3 };
 // multiple of it.
 constexpr int kNB = 64;
 
-// 1: the update kernels read and write their C tiles with non-temporal
-// (streaming) accesses, so the panel data they reuse stays in L2.
-#ifndef EXA_C_NONTEMPORAL
-#define EXA_C_NONTEMPORAL 0
-#endif
-
 // Device view of a tiled matrix: square ts x ts tiles, column-major inside
 // a tile with leading dimension ts.  Lower-packed matrices store only the
 // tiles m >= n, column after column; full matrices store mt*nt tiles

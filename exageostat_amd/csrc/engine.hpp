@@ -45,6 +45,8 @@ class TileMatrix {
   // change any tile's precision without reallocating.
   void allocate(int m, int n, int ts, bool lower, const std::vector<uint8_t>& prec = {}, bool fp64_slots = false);
   void set_prec(const std::vector<uint8_t>& prec);
+  // Every tile's slot can hold the precision `prec` gives it.
+  bool fits(const std::vector<uint8_t>& prec) const;
   const std::vector<uint8_t>& prec_map() const { return h_prec_; }
   void release();
   bool allocated() const { return pool_ != nullptr; }
@@ -67,6 +69,7 @@ class TileMatrix {
   bool fp64_slots_ = false;
   std::vector<void*> h_ptr_;
   std::vector<uint8_t> h_prec_;
+  std::vector<uint8_t> h_slot_;  // bytes per entry of each tile's slot
   void** d_ptr_ = nullptr;
   uint8_t* d_prec_ = nullptr;
 };

@@ -117,9 +117,13 @@ void launch_split_panel(const TileView& v, int k, uint16_t* split, hipStream_t s
 // the lo*lo product when lolo), [2] far tiles (BF16 MFMA).  nblocks counts
 // tile pairs; the BF16 classes read their operands from the split panel of
 // step k.
+// The three precision classes of one fused update launch, in the order each
+// XCD walks them: slot c holds the pairs of class cls[c] (0: FP64 tiles,
+// 1: FP32 tiles, 2: far tiles).
 struct FusedSpans {
   const int* pairs[3];
   long nblocks[3];
+  int cls[3];
 };
 void launch_update_fused(const TileView& v, int k, const FusedSpans& sp, const uint16_t* split, bool lolo,
                          hipStream_t s);

@@ -58,11 +58,7 @@ __device__ __forceinline__ void acc_add_lp(fx4 (&acc)[C::FM][C::FN], T* Cp, long
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int r = wm * C::WM + i * 16 + frag_row_lp(lane, v);
-#if EXA_C_NONTEMPORAL
-        acc[i][j][v] = (float)__builtin_nontemporal_load(&G[r + (long)c * ldc]) + sign * acc[i][j][v];
-#else
         acc[i][j][v] = (float)G[r + (long)c * ldc] + sign * acc[i][j][v];
-#endif
       }
     }
 #pragma unroll
@@ -73,11 +69,7 @@ __device__ __forceinline__ void acc_add_lp(fx4 (&acc)[C::FM][C::FN], T* Cp, long
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int r = wm * C::WM + i * 16 + frag_row_lp(lane, v);
-#if EXA_C_NONTEMPORAL
-        __builtin_nontemporal_store((T)acc[i][j][v], &G[r + (long)c * ldc]);
-#else
         G[r + (long)c * ldc] = (T)acc[i][j][v];
-#endif
       }
     }
 }

@@ -27,7 +27,7 @@
 namespace exa {
 
 struct MixedState {
-  TileMatrix Cm;       // mixed factor, every tile in an FP64-sized slot
+  TileMatrix Cm;       // mixed factor, each tile in a slot of its precision
   ClassLists lists;    // update pairs per precision class
   uint16_t* split = nullptr;  // two panels in split BF16 form
   long split_tiles = 0;
@@ -170,7 +170,14 @@ double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric,
   hipStream_t s1 = ctx.s1;
 
   const double t0 = wall_seconds();
-  if (!st.Cm.allocated()) st.Cm.allocate(N, N, ts, true, {}, true);
+  // Each tile's slot holds exactly its precision (FP32 tiles take 4 bytes
+  // per entry); a map that needs a larger slot somewhere than the current
+  // allocation gives it repacks the factor's pool.
+  std::vector<uint8_t> map = precision_map(st, ctx, kid, theta, metric, opt, mt, ts);
+  if (!st.Cm.fits(map) || st.Cm.m != N || st.Cm.ts != ts) {
+    st.Cm.release();
+    st.Cm.allocate(N, N, ts, true, map, false);
+  }
   st.x.allocate(N, padded);
   st.r.allocate(N, padded);
   if (st.part_len < 2L * mt * padded) {
@@ -185,7 +192,7 @@ double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric,
     EXA_HIP(hipMalloc(&st.split, sizeof(uint16_t) * 4L * mt * ts * ts));
     st.split_tiles = mt;
   }
-  st.Cm.set_prec(precision_map(st, ctx, kid, theta, metric, opt, mt, ts));
+  st.Cm.set_prec(map);
   st.lists.build(st.Cm);
   st.lists.split[0] = st.split;
   st.lists.split[1] = st.split + 2L * mt * ts * ts;

@@ -226,9 +226,12 @@ static void launch_class_bs(const TileView& v, int k, const int* pairs, long npa
 
 template <int BS, int F32CLS>
 static void launch_per_class(const TileView& v, int k, const FusedSpans& sp, const uint16_t* split, hipStream_t s) {
-  if (sp.nblocks[0] > 0) launch_class_bs<kF64, BS>(v, k, sp.pairs[0], sp.nblocks[0], split, s);
-  if (sp.nblocks[1] > 0) launch_class_bs<F32CLS, BS>(v, k, sp.pairs[1], sp.nblocks[1], split, s);
-  if (sp.nblocks[2] > 0) launch_class_bs<kF32B, BS>(v, k, sp.pairs[2], sp.nblocks[2], split, s);
+  for (int c = 0; c < 3; ++c) {
+    if (sp.nblocks[c] <= 0) continue;
+    if (sp.cls[c] == 0) launch_class_bs<kF64, BS>(v, k, sp.pairs[c], sp.nblocks[c], split, s);
+    else if (sp.cls[c] == 1) launch_class_bs<F32CLS, BS>(v, k, sp.pairs[c], sp.nblocks[c], split, s);
+    else launch_class_bs<kF32B, BS>(v, k, sp.pairs[c], sp.nblocks[c], split, s);
+  }
 }
 
 // The trailing update of one step over all three precision classes in one
@@ -282,8 +285,9 @@ __global__ void __launch_bounds__((ClassCfg<kF64, BS>::C::NT))
     const int m = pk >> 16;
     const int n = pk & 0xFFFF;
     if (m == n && bi < bj) continue;
-    if (c == 0) class_update<kF64, BS>(v, k, m, n, bi, bj, split, lds);
-    else if (c == 1) class_update<F32CLS, BS>(v, k, m, n, bi, bj, split, lds);
+    const int cl = sp.cls[c];
+    if (cl == 0) class_update<kF64, BS>(v, k, m, n, bi, bj, split, lds);
+    else if (cl == 1) class_update<F32CLS, BS>(v, k, m, n, bi, bj, split, lds);
     else class_update<kF32B, BS>(v, k, m, n, bi, bj, split, lds);
   }
 }
@@ -339,8 +343,9 @@ static void launch_fused(const TileView& v, int k, const FusedSpans& sp, const u
 
 void launch_update_fused(const TileView& v, int k, const FusedSpans& sp, const uint16_t* split, bool lolo,
                          hipStream_t s) {
-  if ((sp.nblocks[1] > 0 || sp.nblocks[2] > 0) && split == nullptr)
-    throw std::invalid_argument("BF16 update classes need the split panel");
+  for (int c = 0; c < 3; ++c)
+    if (sp.nblocks[c] > 0 && sp.cls[c] != 0 && split == nullptr)
+      throw std::invalid_argument("BF16 update classes need the split panel");
   if (lolo) launch_fused<kF32Q>(v, k, sp, split, s);
   else launch_fused<kF32>(v, k, sp, split, s);
 }
