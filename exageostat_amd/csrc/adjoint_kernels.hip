@@ -47,24 +47,28 @@ __global__ void __launch_bounds__(C::NT) k_trmm_panel(TileView v, int k, const d
 }
 
 // Z_mk = - sum_{j>k} Z(m,j) P_j for m in (k, mt).  With nsplit > 1 the
-// sum over j is cut into nsplit contiguous ranges; work-group set s writes
-// its partial sum of range s to slab s of W (below tiles of ts x ts each,
-// tile m - k - 1 at W + (s * below + m - k - 1) ts^2) and k_sum_slabs adds
-// the slabs into Z_mk in a fixed order.  More, smaller work units fill the
-// GPU's CUs evenly when (mt - k - 1) * (ts / BM)^2 blocks alone would leave
-// a partial last round.
-// U (optional, U.ptr null when absent): tile (j, m), j > m, holds Z_jm^T,
-// written by the epilogues of the column stages before it is read, so that
-// every operand is read in its plain (row-contiguous) layout.
-template <class C>
-__global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, const double* P, double* W, int nsplit,
-                                                         TileView U) {
-  __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
+// sum over j is cut into nsplit contiguous ranges; work unit (s, m, block)
+// writes its partial sum of range s to slab s of W (below tiles of ts x ts
+// each, tile m - k - 1 at W + (s * below + m - k - 1) ts^2) and k_sum_slabs
+// adds the slabs into Z_mk in a fixed order.  More, smaller work units fill
+// the GPU's CUs evenly when (mt - k - 1) * (ts / BM)^2 units alone would
+// leave a partial last round.
+// HAS_U: tile (j, m), j > m, of U holds Z_jm^T, written by the epilogues of
+// the column stages before it is read, so every operand is read in its
+// plain (row-contiguous) layout and each j step is the same GEMM on
+// different pointers (one instance of the GEMM body: the kernel keeps its
+// accumulators in registers).  Without U, Z_jm is read transposed.
+// Persistent launch, as in the Cholesky update: work-group w runs on XCD
+// w % nx and the XCDs take contiguous slices of the unit list (units of one
+// range s and row m share their A tiles and every P_j of the range).
+template <class C, bool HAS_U>
+__device__ __forceinline__ void takahashi_unit(const TileView& v, int k, const double* P, double* W, int nsplit,
+                                               const TileView& U, long u, double* lds) {
   const int qm = v.ts / C::BM, qn = v.ts / C::BN;
   const long below = v.mt - k - 1;
   const long per_split = below * qm * qn;
-  const int sp = (int)(blockIdx.x / per_split);
-  const long b = blockIdx.x - (long)sp * per_split;
+  const int sp = (int)(u / per_split);
+  const long b = u - (long)sp * per_split;
   const int m = k + 1 + (int)(b / (qm * qn));
   const int rem = (int)(b % (qm * qn));
   const int bi = rem % qm, bj = rem / qm;
@@ -74,26 +78,38 @@ __global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, cons
   const int j_hi = j_lo + chunk < v.mt ? j_lo + chunk : v.mt;
   dx4 acc[C::FM][C::FN];
   zero_acc<C>(acc);
+#pragma unroll 1
   for (int j = j_lo; j < j_hi; ++j) {
     const double* B = P + (long)j * ts * ts + bj * C::BN;  // B(c, kk) = P_j^T[bj*BN + c, kk]
-    if (j <= m) {
-      const double* A = tile_f64(v, m, j) + bi * C::BM;
+    if constexpr (HAS_U) {
+      // A(r, kk) = Z(m, j)[bi*BM + r, kk]: tile (m, j) of Z for j <= m, Z_jm^T from U for j > m
+      const double* A = (j <= m ? tile_f64(v, m, j) : tile_f64(U, j, m)) + bi * C::BM;
       gemm_nt_acc<C>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
-    } else if (U.ptr) {
-      const double* A = tile_f64(U, j, m) + bi * C::BM;  // A(r, kk) = Z_jm^T[bi*BM + r, kk]
-      gemm_nt_acc<C>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
+    } else if (j <= m) {
+      gemm_nt_acc<C>(acc, tile_f64(v, m, j) + bi * C::BM, ts, B, ts, (int)ts, lds, -1.0);
     } else {
       const double* A = tile_f64(v, j, m) + (long)bi * C::BM * ts;  // A(r, kk) = Z_jm[kk, bi*BM + r]
       gemm_nt_acc<C, false, double, double, true>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
     }
   }
-  if (nsplit > 1) {
-    acc_update<C>(acc, W + ((long)sp * below + (m - k - 1)) * ts * ts + bi * C::BM + (long)bj * C::BN * ts, ts,
-                  0.0);
-    return;
-  }
-  if (U.ptr) acc_store_t<C>(acc, tile_f64(U, m, k) + bj * C::BN + (long)bi * C::BM * ts, ts);
-  acc_update<C>(acc, tile_f64(v, m, k) + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
+  double* out = nsplit > 1 ? W + ((long)sp * below + (m - k - 1)) * ts * ts : tile_f64(v, m, k);
+  acc_update<C>(acc, out + bi * C::BM + (long)bj * C::BN * ts, ts, 0.0);
+  if constexpr (HAS_U)
+    if (nsplit == 1) acc_store_t<C>(acc, tile_f64(U, m, k) + bj * C::BN + (long)bi * C::BM * ts, ts);
+}
+
+template <class C, bool HAS_U>
+__global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, const double* P, double* W, int nsplit,
+                                                         TileView U, long units, int nx) {
+  __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
+  const int w = blockIdx.x;
+  const int x = w % nx;
+  const int gx = gridDim.x / nx;
+  const long per = (units + nx - 1) / nx;
+  const long start = (long)x * per;
+  const long end = start + per < units ? start + per : units;
+#pragma unroll 1
+  for (long u = start + w / nx; u < end; u += gx) takahashi_unit<C, HAS_U>(v, k, P, W, nsplit, U, u, lds);
 }
 
 // Z_mk = sum_s W[s][m - k - 1] for m in (k, mt), s ascending, and (with
@@ -169,14 +185,15 @@ __global__ void k_sum_tiles(const double* Q, int cnt, long len, double* out) {
 // units fill whole rounds of the concurrent work-groups best; each extra
 // range costs a write and a read of its slab, counted as 2% per range.
 template <class C>
-static int takahashi_splits(const TileView& v, int k, int max_split) {
+static int takahashi_splits(const TileView& v, int k, int max_split, int* slots_out) {
   static int slots = 0;
   if (!slots) {
     int dev = 0, cus = 0, occ = 1;
     EXA_HIP(hipGetDevice(&dev));
     EXA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    EXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_takahashi_col<C>, C::NT, 0));
+    EXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_takahashi_col<C, true>, C::NT, 0));
     slots = cus * (occ > 0 ? occ : 1);
+    if (slots >= 8) slots -= slots % 8;
   }
   const long per = (long)(v.ts / C::BM) * (v.ts / C::BN);
   const long below = v.mt - k - 1;
@@ -192,6 +209,7 @@ static int takahashi_splits(const TileView& v, int k, int max_split) {
       best = sp;
     }
   }
+  *slots_out = slots;
   return best;
 }
 
@@ -203,9 +221,21 @@ static void takahashi_cfg(const TileView& v, int k, const double* X, double* P, 
   if (stage == 0 && below > 0)
     hipLaunchKernelGGL(k_trmm_panel<C>, dim3((unsigned)(below * per)), dim3(C::NT), 0, s, v, k, X, P);
   if (stage == 1 && below > 0) {
-    const int ns = W ? takahashi_splits<C>(v, k, max_split) : 1;
-    hipLaunchKernelGGL(k_takahashi_col<C>, dim3((unsigned)(below * per * ns)), dim3(C::NT), 0, s, v, k,
-                       (const double*)P, W, ns, U);
+    int slots = 0;
+    const int ns = takahashi_splits<C>(v, k, W ? max_split : 1, &slots);
+    const long units = below * per * ns;
+    // persistent grid: at most one round of resident work-groups, a
+    // multiple of the 8 XCDs when it is that large
+    long grid = units < slots ? units : slots;
+    int nx = 8;
+    if (grid >= 8) grid -= grid % 8;
+    else nx = 1;
+    if (U.ptr)
+      hipLaunchKernelGGL((k_takahashi_col<C, true>), dim3((unsigned)grid), dim3(C::NT), 0, s, v, k,
+                         (const double*)P, W, ns, U, units, nx);
+    else
+      hipLaunchKernelGGL((k_takahashi_col<C, false>), dim3((unsigned)grid), dim3(C::NT), 0, s, v, k,
+                         (const double*)P, W, ns, U, units, nx);
     if (ns > 1) {
       const long q = v.ts / 64;
       hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)(below * q * q)), dim3(256), 0, s, v, k, (const double*)W, ns,

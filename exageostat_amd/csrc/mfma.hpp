@@ -240,23 +240,25 @@ __device__ __forceinline__ void acc_update(dx4 (&acc)[C::FM][C::FN], T* Cp, long
   }
 }
 
-// Cp[c + r*ldc] = acc(r, c): the block stored transposed.
+// Cp[c + r*ldc] = acc(r, c): the block stored transposed.  Each lane keeps
+// one base address; the fragment offsets are uniform (a multiple of ldc plus
+// a constant), so the stores need no per-element address registers next to
+// the live accumulators.
 template <class C>
 __device__ __forceinline__ void acc_store_t(const dx4 (&acc)[C::FM][C::FN], double* Cp, long ldc) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int wm = wave / C::WGN;
   const int wn = wave - wm * C::WGN;
+  typedef __attribute__((address_space(1))) double gD;
+  gD* G = (gD*)(uintptr_t)(Cp + wn * C::WN + frag_col(lane) + (long)(wm * C::WM + frag_row(lane, 0)) * ldc);
 #pragma unroll
   for (int i = 0; i < C::FM; ++i)
 #pragma unroll
-    for (int j = 0; j < C::FN; ++j) {
-      const int c = wn * C::WN + j * 16 + frag_col(lane);
+    for (int v = 0; v < 4; ++v) {
+      const long ro = (long)(i * 16 + 4 * v) * ldc;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = wm * C::WM + i * 16 + frag_row(lane, v);
-        Cp[c + (long)r * ldc] = acc[i][j][v];
-      }
+      for (int j = 0; j < C::FN; ++j) G[ro + j * 16] = acc[i][j][v];
     }
 }
 
