@@ -174,15 +174,24 @@ def _fp64_vs_mixed(m, theta, steps: int, variants: dict) -> dict:
 
 
 # Mixed option sets measured at the larger configurations: the benchmark's
-# set, and the same with the lo*lo product kept in the FP32 tiles.
-VARIANTS = {"mixed": MIXED, "mixed_lolo": dict(MIXED, lolo=True)}
+# set, the same with the lo*lo product kept in the FP32 tiles, and per
+# configuration the set that keeps |dl|/|l| <= 1e-6: the log-det error left
+# with lo*lo is the FP32 storage of the tiles (a three-term split of the
+# products, 24 bits, measured the same error), so the remaining lever is
+# FP64 storage for the most correlated tiles.  Up to 8 refinement steps (the
+# refinement stops once a step changes z^T Sigma^-1 z by <= 1e-10).
+_M8 = dict(MIXED, ir_steps=8)
+VARIANTS = {"mixed": _M8, "mixed_lolo": dict(_M8, lolo=True)}
+VARIANTS_C3_ROUGH = dict(VARIANTS, mixed_lolo_fp64corr_0_7=dict(_M8, lolo=True, fp64_corr=0.7))
+VARIANTS_C5 = dict(VARIANTS, mixed_lolo_nobf16_fp64corr_0_3=dict(_M8, lolo=True, bf16=False, fp64_corr=0.3))
 
 
 def run_c3(steps: int) -> None:
     """Mixed-precision accuracy and speed at n = 160,000 (400 x 400 jittered
     grid), ts = 1,024, for theta = (1, 0.03, 1.0) and (1, 0.3, 0.5); z drawn
     at each theta; FP64 diagonal tiles, FP32 tiles updated with split BF16,
-    BF16 far tiles (correlation < 0.01), 3 FP64 refinement steps."""
+    BF16 far tiles (correlation < 0.01), up to 8 FP64 refinement steps; at
+    theta = (1, 0.3, 0.5) also the tiles of correlation >= 0.7 in FP64."""
     import exageostat_amd
     from exageostat_amd import mle
     n, ts = 160000, 1024
@@ -192,7 +201,7 @@ def run_c3(steps: int) -> None:
     res = []
     for theta in ([1.0, 0.03, 1.0], [1.0, 0.3, 0.5]):
         m.zvg(theta, e)
-        r = _fp64_vs_mixed(m, theta, steps, VARIANTS)
+        r = _fp64_vs_mixed(m, theta, steps, VARIANTS if theta[1] < 0.1 else VARIANTS_C3_ROUGH)
         m.precision = 2
         m.set_mixed(**MIXED)
         m.loglik(theta)
@@ -224,7 +233,7 @@ def run_c5(steps: int) -> None:
     m.zvg(theta, e)
     z = m.get_z()
     out = {"config": "C5: bivariate parsimonious, 50000 locations x 2", "theta": theta.tolist()}
-    out["fp64_vs_mixed"] = _fp64_vs_mixed(m, theta, steps, VARIANTS)
+    out["fp64_vs_mixed"] = _fp64_vs_mixed(m, theta, steps, VARIANTS_C5)
     t0 = time.perf_counter()
     _, g = m.loglik(theta, grad=True)
     out["grad_s"] = time.perf_counter() - t0
@@ -258,14 +267,18 @@ def run_c5(steps: int) -> None:
     print(json.dumps(out), flush=True)
 
 
-def run_c4(steps: int, max_evals: int = 3000) -> None:
+C4_FITS = ("lbfgsb", "bobyqa_ftol1e-5", "bobyqa_ftol1e-7")
+
+
+def run_c4(steps: int, max_evals: int = 3000, which=C4_FITS) -> None:
     """Adjoint gradient and gradient-based MLE at n = 40,000 (200 x 200
     jittered grid), nugget kernel, theta = (1, 0.1, 0.5, 0.01), ts = 960:
     one l evaluation and one l + gradient, the gradient against central
     differences (h = 1e-6 relative), then L-BFGS-B over the adjoint gradient
     from theta0 = (0.5, 0.05, 1.0, 0.05) against the reference's optimiser
     (BOBYQA over MLE_alg from the lower bounds, ftol_abs 1e-5 and, for a
-    converged comparison, 1e-7), bounds [0.01, 5]."""
+    converged comparison, 1e-7), bounds [0.01, 5].  `which` selects the
+    fits (the three take tens of minutes together)."""
     import exageostat_amd
     from exageostat_amd import mle, optimize
     n, ts = 40000, 960
@@ -299,14 +312,18 @@ def run_c4(steps: int, max_evals: int = 3000) -> None:
     fits = {}
     for name, meth, st, tol in [("lbfgsb", "lbfgsb", start, 5), ("bobyqa_ftol1e-5", "bobyqa", None, 5),
                                 ("bobyqa_ftol1e-7", "bobyqa", None, 7)]:
+        if name not in which:
+            continue
         r = optimize.fit(m, lb, ub, start=st, method=meth, opt_tol=tol, max_evals=max_evals)
         fits[name] = {"theta": r.theta.tolist(), "loglik": r.loglik, "evals": r.evals, "wall_s": r.wall_s,
                       "message": r.message}
         print(f"bench: c4 {name}: {r.wall_s:.1f}s {r.evals} evals theta={np.round(r.theta, 6)} l={r.loglik:.6f}",
               file=sys.stderr, flush=True)
     m.close()
-    a = np.array(fits["lbfgsb"]["theta"])
     for k in ("bobyqa_ftol1e-5", "bobyqa_ftol1e-7"):
+        if k not in fits or "lbfgsb" not in fits:
+            continue
+        a = np.array(fits["lbfgsb"]["theta"])
         b = np.array(fits[k]["theta"])
         fits[k]["theta_max_rel_diff_vs_lbfgsb"] = float(np.max(np.abs(a - b) / np.abs(b)))
         fits[k]["wall_ratio_vs_lbfgsb"] = fits[k]["wall_s"] / fits["lbfgsb"]["wall_s"]
@@ -359,6 +376,7 @@ def main() -> None:
     ap.add_argument("--cpu-sample", type=int, default=16000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="time the FP64 path only")
+    ap.add_argument("--fits", default=",".join(C4_FITS), help="--config c4: comma-separated fits to run")
     ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "mloe"],
                     help="c2 (default): the benchmark line; c3, c4, c5: the other configurations' measurements; "
                          "mloe: MLOE/MMOM at n = 90,000")
@@ -384,7 +402,7 @@ def main() -> None:
         run_c3(args.steps)
         return
     if args.config == "c4":
-        run_c4(args.steps)
+        run_c4(args.steps, which=args.fits.split(","))
         return
     if args.config == "c5":
         run_c5(args.steps)
