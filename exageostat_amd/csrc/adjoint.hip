@@ -90,9 +90,9 @@ void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, 
 // Scratch of the Takahashi sweep and of the dSigma reductions, sized for
 // ctx.C and kept across calls.
 static AdjointState& adjoint_buffers(Context& ctx) {
+  TileMatrix& A = ctx.fp64("the gradient and the Fisher information");
   if (!ctx.adjoint) ctx.adjoint = new AdjointState();
   AdjointState& st = *ctx.adjoint;
-  TileMatrix& A = ctx.C;
   const long tile_elems = (long)A.ts * A.ts;
   if (st.tiles < A.mt) {
     if (st.P) (void)hipFree(st.P);
@@ -161,7 +161,7 @@ double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int 
     return ll;
   }
   AdjointState& st = adjoint_buffers(ctx);
-  TileMatrix& A = ctx.C;
+  TileMatrix& A = ctx.fp64("the gradient and the Fisher information");
   const TileView v = A.view();
   hipStream_t s = ctx.s1;
 
@@ -232,7 +232,7 @@ int fisher_information(Context& ctx, KernelId kid, const double* theta, int metr
   ctx.loglik(kid, theta, metric, &logdet, &dotp, &inf);
   if (inf != 0) return inf;
   AdjointState& st = adjoint_buffers(ctx);
-  TileMatrix& A = ctx.C;
+  TileMatrix& A = ctx.fp64("the gradient and the Fisher information");
   hipStream_t s = ctx.s1;
   if (!st.D.allocated() || st.D.mt != A.mt || st.D.ts != A.ts) {
     st.D.release();

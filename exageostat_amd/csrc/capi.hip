@@ -72,6 +72,23 @@ struct DescImpl {
   }
 };
 
+// The payload of a descriptor: std::invalid_argument for a null descriptor
+// and for a tile-matrix descriptor whose matrix is not allocated (the FP64
+// covariance of a lean context).
+DescImpl* impl_of(const CHAM_desc_t* d) {
+  if (!d || !d->mat) throw std::invalid_argument("null descriptor");
+  auto* impl = static_cast<DescImpl*>(d->mat);
+  if (impl->tm && !impl->tm->allocated())
+    throw std::invalid_argument("the descriptor's tile matrix is not allocated (descC of a lean context)");
+  return impl;
+}
+
+// The tile matrix of a descriptor payload (not a vector or scalar one).
+TileMatrix& need_tm(DescImpl* impl) {
+  if (!impl->tm) throw std::invalid_argument("the descriptor is not a tile matrix");
+  return *impl->tm;
+}
+
 CHAM_desc_t* make_desc(TileMatrix* tm, bool owns) {
   auto* d = new CHAM_desc_t();
   auto* impl = new DescImpl();
@@ -660,7 +677,7 @@ int EXAGEOSTAT_Sequence_Destroy(RUNTIME_sequence_t* sequence) {
 
 int EXAGEOSTAT_Lapack_to_Tile(const double* A, int lda, CHAM_desc_t* desc) {
   return guarded([&]() {
-    auto* impl = static_cast<DescImpl*>(desc->mat);
+    auto* impl = impl_of(desc);
     if (impl->host) {
       for (int j = 0; j < desc->n; ++j)
         for (int i = 0; i < desc->m; ++i) impl->host[i + (long)j * desc->m] = A[i + (long)j * lda];
@@ -677,7 +694,7 @@ int EXAGEOSTAT_Lapack_to_Tile(const double* A, int lda, CHAM_desc_t* desc) {
 
 int EXAGEOSTAT_Tile_to_Lapack(CHAM_desc_t* desc, double* A, int lda) {
   return guarded([&]() {
-    auto* impl = static_cast<DescImpl*>(desc->mat);
+    auto* impl = impl_of(desc);
     if (impl->host) {
       for (int j = 0; j < desc->n; ++j)
         for (int i = 0; i < desc->m; ++i) A[i + (long)j * lda] = impl->host[i + (long)j * desc->m];
@@ -696,17 +713,18 @@ int EXAGEOSTAT_Tile_to_Lapack(CHAM_desc_t* desc, double* A, int lda) {
 int EXAGEOSTAT_dpotrf_Tile(CHAM_enum uplo, CHAM_desc_t* A) {
   return guarded([&]() {
     if (uplo != ChamLower) throw std::invalid_argument("only ChamLower is supported");
-    auto* impl = static_cast<DescImpl*>(A->mat);
-    if (!impl->tm->lower) throw std::invalid_argument("descriptor must be lower-packed");
-    return tile_potrf(*impl->tm, rt().ws, rt().s1, rt().s2, true);
+    auto* impl = impl_of(A);
+    TileMatrix& tm = need_tm(impl);
+    if (!tm.lower) throw std::invalid_argument("descriptor must be lower-packed");
+    return tile_potrf(tm, rt().ws, rt().s1, rt().s2, true);
   }, -1);
 }
 
 int EXAGEOSTAT_dpotri_Tile(CHAM_enum uplo, CHAM_desc_t* A) {
   return guarded([&]() {
     if (uplo != ChamLower) throw std::invalid_argument("only ChamLower is supported");
-    auto* impl = static_cast<DescImpl*>(A->mat);
-    TileMatrix& tm = *impl->tm;
+    auto* impl = impl_of(A);
+    TileMatrix& tm = need_tm(impl);
     if (!tm.lower) throw std::invalid_argument("descriptor must be lower-packed");
     const size_t bytes = sizeof(double) * (size_t)tm.ts * tm.ts * tm.mt;
     double *P = nullptr, *Q = nullptr;
@@ -731,8 +749,8 @@ int EXAGEOSTAT_dpotri_Tile(CHAM_enum uplo, CHAM_desc_t* A) {
 
 int EXAGEOSTAT_dtrsv_Tile(CHAM_enum trans, CHAM_desc_t* L, double* b) {
   return guarded([&]() {
-    auto* impl = static_cast<DescImpl*>(L->mat);
-    TileMatrix& tm = *impl->tm;
+    auto* impl = impl_of(L);
+    TileMatrix& tm = need_tm(impl);
     DeviceVector v;
     v.allocate(tm.m, (long)tm.mt * tm.ts);
     copy_vec_to_device(v, b, tm.m, rt().s1);
@@ -745,9 +763,9 @@ int EXAGEOSTAT_dtrsv_Tile(CHAM_enum trans, CHAM_desc_t* L, double* b) {
 
 int EXAGEOSTAT_dtrsv_Tile_noalloc(CHAM_enum trans, CHAM_desc_t* L, double* b) {
   return guarded([&]() {
-    auto* impl = static_cast<DescImpl*>(L->mat);
+    auto* impl = impl_of(L);
     if (!impl || !impl->tm) throw std::invalid_argument("L must be a tile-matrix descriptor");
-    TileMatrix& tm = *impl->tm;
+    TileMatrix& tm = need_tm(impl);
     const long padded = (long)tm.mt * tm.ts;
     if (padded > kNoAllocRows || tm.ts > kNoAllocTs)
       throw std::invalid_argument("EXAGEOSTAT_dtrsv_Tile_noalloc: mt * ts = " + std::to_string(padded) +
@@ -784,8 +802,8 @@ int EXAGEOSTAT_MLE_dcmg_Tile_Async(CHAM_enum uplo, CHAM_desc_t* descA, location*
     if (sequence && sequence->status != 0) return -2;
     const KernelId kid = kernel_from_name(kernel_fun);
     if (kid == kUnknownKernel) throw std::invalid_argument(std::string("unsupported kernel: ") + (kernel_fun ? kernel_fun : "(null)"));
-    auto* impl = static_cast<DescImpl*>(descA->mat);
-    TileMatrix& tm = *impl->tm;
+    auto* impl = impl_of(descA);
+    TileMatrix& tm = need_tm(impl);
     const long nr = tm.m / kernel_vars(kid);
     const long nc = tm.n / kernel_vars(kid);
     hipStream_t s = seq_stream(sequence);
@@ -807,10 +825,10 @@ int EXAGEOSTAT_MLE_dmdet_Tile_Async(CHAM_desc_t* descA, RUNTIME_sequence_t* sequ
   (void)request;
   return guarded([&]() {
     if (sequence && sequence->status != 0) return -2;
-    auto* impl = static_cast<DescImpl*>(descA->mat);
-    auto* dimpl = static_cast<DescImpl*>(descdet->mat);
+    auto* impl = impl_of(descA);
+    auto* dimpl = impl_of(descdet);
     if (!dimpl->host) throw std::invalid_argument("descdet must be a host scalar descriptor");
-    TileMatrix& tm = *impl->tm;
+    TileMatrix& tm = need_tm(impl);
     hipStream_t s = seq_stream(sequence);
     if (sequence && sequence->impl) {
       auto* simpl = static_cast<SeqImpl*>(sequence->impl);
@@ -844,7 +862,7 @@ void EXAGEOSTAT_dmle_Call(MLE_data* data, int ncores, int gpus, int dts, int p_g
   (void)q_grid;
   guarded([&]() {
     if (data->gpu) MLE_Finalize(data);
-    auto* ctx = new Context(N, dts, nZobs, nZmiss);
+    auto* ctx = new Context(N, dts, nZobs, nZmiss, data->mp_lean != 0);
     data->gpu = ctx;
     data->descC = make_desc(&ctx->C, false);
     data->descZ = make_vec_desc(&ctx->Z, dts);
@@ -1025,8 +1043,9 @@ void EXAGEOSTAT_MLE_dzvg_Tile(MLE_data* data, double* Nrand, double* initial_the
     if (kid == kUnknownKernel) throw std::invalid_argument("unsupported kernel");
     const long nloc = n / kernel_vars(kid);
     ctx->l1.upload(data->l1.x, data->l1.y, nloc, ctx->s1, data->l1.z);
-    generate_covariance(ctx->C, kid, initial_theta, ctx->l1, ctx->l1, metric_of(data->dm), ctx->s1);
-    const int info = tile_potrf(ctx->C, ctx->ws, ctx->s1, ctx->s2, true);
+    TileMatrix& C = ctx->fp64("drawing z (MLE_zvg)");
+    generate_covariance(C, kid, initial_theta, ctx->l1, ctx->l1, metric_of(data->dm), ctx->s1);
+    const int info = tile_potrf(C, ctx->ws, ctx->s1, ctx->s2, true);
     if (info != 0) throw std::runtime_error("dzvg: covariance is not positive definite");
     copy_vec_to_device(ctx->work, Nrand, n, ctx->s1);
     ctx->gemv_part.allocate((long)ctx->C.mt * ctx->C.mt * ctx->C.ts, (long)ctx->C.mt * ctx->C.mt * ctx->C.ts);

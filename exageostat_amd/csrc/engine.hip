@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 
 namespace exa {
 
@@ -269,8 +270,12 @@ void DeviceLocations::release() {
 // ------------------------------------------------------------- generation --
 void generate_covariance(TileMatrix& A, KernelId kid, const double* theta, const DeviceLocations& rows,
                          const DeviceLocations& cols, int metric, hipStream_t s) {
-  TileView v = A.view();
-  const long ntiles = A.num_tiles();
+  if (!A.allocated()) throw std::invalid_argument("covariance generation into an unallocated tile matrix");
+  generate_covariance_tiles(A.view(), kid, theta, rows, cols, metric, 0, A.num_tiles(), s);
+}
+
+void generate_covariance_tiles(const TileView& v, KernelId kid, const double* theta, const DeviceLocations& rows,
+                               const DeviceLocations& cols, int metric, long p0, long ntiles, hipStream_t s) {
   const bool three_d = rows.dz && cols.dz && kid != kSpaceTime;
   if (three_d && metric == 1)
     throw std::invalid_argument("great-circle distances are 2-D only (locations carry a z coordinate)");
@@ -288,9 +293,9 @@ void generate_covariance(TileMatrix& A, KernelId kid, const double* theta, const
     P.nugget = (kid == kNugget) ? theta[3] : 0.0;
     P.con = 1.0 / (std::pow(2.0, theta[2] - 1.0) * std::tgamma(theta[2]));
     knu_prepare(theta[2], P.ko);
-    launch_gen_matern(v, P, 0, ntiles, s);
+    launch_gen_matern(v, P, p0, ntiles, s);
   } else if (kid == kBivariate || kid == kBivariateBlock) {
-    if (kid == kBivariate && A.ts % 2 != 0) throw std::invalid_argument("bivariate kernel needs an even tile size");
+    if (kid == kBivariate && v.ts % 2 != 0) throw std::invalid_argument("bivariate kernel needs an even tile size");
     BivariateParams P;
     P.x1 = rows.dx;
     P.y1 = rows.dy;
@@ -316,7 +321,7 @@ void generate_covariance(TileMatrix& A, KernelId kid, const double* theta, const
     knu_prepare(nu1, P.ko11);
     knu_prepare(nu2, P.ko22);
     knu_prepare(nu12, P.ko12);
-    launch_gen_bivariate(v, P, 0, ntiles, kid == kBivariateBlock, s);
+    launch_gen_bivariate(v, P, p0, ntiles, kid == kBivariateBlock, s);
   } else if (kid == kSpaceTime) {
     if (!rows.dz || !cols.dz) throw std::invalid_argument("space-time kernel: locations need z (time)");
     SpaceTimeParams P;
@@ -335,7 +340,7 @@ void generate_covariance(TileMatrix& A, KernelId kid, const double* theta, const
     P.delta = theta[6];
     P.con = 1.0 / (std::pow(2.0, theta[2] - 1.0) * std::tgamma(theta[2]));
     knu_prepare(theta[2], P.ko);
-    launch_gen_spacetime(v, P, 0, ntiles, s);
+    launch_gen_spacetime(v, P, p0, ntiles, s);
   } else {
     throw std::invalid_argument("unsupported kernel");
   }
@@ -660,18 +665,29 @@ double device_dot(const double* a, const double* b, long n, double* d_partial, h
 }
 
 // ---------------------------------------------------------------- Context --
-Context::Context(int N_, int ts_, int nZobs, int nZmiss) : N(N_), ts(ts_) {
+Context::Context(int N_, int ts_, int nZobs, int nZmiss, bool lean_) : N(N_), ts(ts_), lean(lean_) {
   (void)nZobs;
   (void)nZmiss;
+  if (N <= 0) throw std::invalid_argument("matrix size must be positive");
+  if (ts <= 0 || ts % kNB != 0 || ts > 1024)
+    throw std::invalid_argument("tile size must be a multiple of 64 and at most 1024");
+  mt = (int)(((long)N + ts - 1) / ts);
   EXA_HIP(hipStreamCreateWithFlags(&s1, hipStreamNonBlocking));
   EXA_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-  C.allocate(N, N, ts, true);
-  const long padded = (long)C.mt * ts;
+  if (!lean) C.allocate(N, N, ts, true);
+  const long padded = (long)mt * ts;
   Z.allocate(N, padded);
   Zcpy.allocate(N, padded);
   work.allocate(N, padded);
-  ws.ensure(C.mt, ts);
-  ensure_partial(padded / 4096 + C.mt + 16);
+  ws.ensure(mt, ts);
+  ensure_partial(padded / 4096 + mt + 16);
+}
+
+TileMatrix& Context::fp64(const char* what) {
+  if (lean)
+    throw std::invalid_argument(std::string(what) +
+                                " needs the FP64 covariance, which a lean context (mp_lean = 1) does not hold");
+  return C;
 }
 
 Context::~Context() {
@@ -695,6 +711,7 @@ void Context::ensure_partial(long n) {
 double Context::loglik(KernelId kid, const double* theta, int metric, double* logdet_out, double* dotp_out,
                        int* info, int band) {
   if (band < 0) throw std::invalid_argument("diag_thick must be positive");
+  fp64("the FP64 likelihood");
   const bool banded = band > 0 && band < C.mt;
   const double t0 = wall_seconds();
   EXA_HIP(hipMemcpyAsync(work.d, Zcpy.d, sizeof(double) * work.padded, hipMemcpyDeviceToDevice, s1));

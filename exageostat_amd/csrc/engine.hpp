@@ -108,6 +108,10 @@ struct DeviceLocations {
 // Covariance generation for a kernel family and theta.
 void generate_covariance(TileMatrix& A, KernelId kid, const double* theta, const DeviceLocations& rows,
                          const DeviceLocations& cols, int metric, hipStream_t s);
+// The tiles p0 .. p0 + ntiles - 1 of v's enumeration (lower: p = m (m + 1) / 2
+// + n, row by row), each stored in its tile's precision.
+void generate_covariance_tiles(const TileView& v, KernelId kid, const double* theta, const DeviceLocations& rows,
+                               const DeviceLocations& cols, int metric, long p0, long ntiles, hipStream_t s);
 
 // Workspace of the tile Cholesky (inverses of 64x64 diagonal blocks, info).
 class CholWorkspace {
@@ -197,11 +201,17 @@ void destroy_mloe(MloeState* m);
 // Everything one MLE_data needs on the device.
 class Context {
  public:
-  explicit Context(int N, int ts, int nZobs, int nZmiss);
+  // lean: no FP64 Sigma is allocated; only the mixed-precision and
+  // single-precision evaluations (which generate straight into their own
+  // factor and regenerate Sigma's tiles for the FP64 refinement) and the
+  // prediction and MLOE paths (own matrices) run on it.
+  explicit Context(int N, int ts, int nZobs, int nZmiss, bool lean = false);
   ~Context();
   hipStream_t s1 = nullptr, s2 = nullptr;
   int N, ts;
-  TileMatrix C;                // Sigma (lower)
+  int mt;                      // tile rows of Sigma
+  bool lean = false;
+  TileMatrix C;                // Sigma (lower); unallocated when lean
   DeviceVector Z, Zcpy, work;  // observations, pristine copy, solve vector
   DeviceLocations l1;
   CholWorkspace ws;
@@ -223,6 +233,8 @@ class Context {
   double loglik(KernelId kid, const double* theta, int metric, double* logdet_out, double* dotp_out, int* info,
                 int band = 0);
   void ensure_partial(long n);
+  // C, or std::invalid_argument naming `what` when the context is lean.
+  TileMatrix& fp64(const char* what);
 };
 
 double wall_seconds();

@@ -53,19 +53,20 @@ __device__ inline double pair_distance(double x1, double y1, double x2, double y
   return sqrt(dx * dx + dy * dy);
 }
 
-// Decode blockIdx.x into (tile m, tile n, sub-block row bi, sub-block col bj).
-__device__ inline bool decode_gen_block(const TileView& v, int first_col, int& m, int& n, int& bi, int& bj) {
+// Decode blockIdx.x into (tile m, tile n, sub-block row bi, sub-block col
+// bj) for the tiles p_off, p_off + 1, ... of the matrix's enumeration
+// (lower matrices: p = m (m + 1) / 2 + n, row by row; full matrices:
+// p = n mt + m).
+__device__ inline bool decode_gen_block(const TileView& v, long p_off, int& m, int& n, int& bi, int& bj) {
   const int q = v.ts / kNB;
   const long b = blockIdx.x;
-  const long p = b / (q * q);
-  const int rem = (int)(b - p * (q * q));
+  const long pb = b / (q * q);
+  const int rem = (int)(b - pb * (q * q));
+  const long p = p_off + pb;
   bj = rem / q;
   bi = rem - bj * q;
   if (v.lower) {
-    int r, c;
-    tri_decode(p, r, c);
-    m = r + first_col;
-    n = c + first_col;
+    tri_decode(p, m, n);
   } else {
     m = (int)(p % v.mt);
     n = (int)(p / v.mt);
@@ -73,9 +74,9 @@ __device__ inline bool decode_gen_block(const TileView& v, int first_col, int& m
   return true;
 }
 
-__global__ void __launch_bounds__(256) k_gen_matern(TileView v, MaternParams P, int first_col) {
+__global__ void __launch_bounds__(256) k_gen_matern(TileView v, MaternParams P, long p_off) {
   int m, n, bi, bj;
-  decode_gen_block(v, first_col, m, n, bi, bj);
+  decode_gen_block(v, p_off, m, n, bi, bj);
   const long t = tile_index(v, m, n);
   void* base = v.ptr[t];
   const uint8_t prec = v.prec ? v.prec[t] : (uint8_t)kF64;
@@ -112,9 +113,9 @@ __global__ void __launch_bounds__(256) k_gen_matern(TileView v, MaternParams P, 
 // row 2p+a, column 2q+b hold cov(variable a at location p, variable b at
 // location q).  Each thread writes one 2x2 block, as the reference's
 // double loop does (core_zcmg.c:1657-1679).
-__global__ void __launch_bounds__(256) k_gen_bivariate(TileView v, BivariateParams P, int first_col) {
+__global__ void __launch_bounds__(256) k_gen_bivariate(TileView v, BivariateParams P, long p_off) {
   int m, n, bi, bj;
-  decode_gen_block(v, first_col, m, n, bi, bj);
+  decode_gen_block(v, p_off, m, n, bi, bj);
   const long t = tile_index(v, m, n);
   void* base = v.ptr[t];
   const uint8_t prec = v.prec ? v.prec[t] : (uint8_t)kF64;
@@ -166,9 +167,9 @@ __global__ void __launch_bounds__(256) k_gen_bivariate(TileView v, BivariatePara
 // quadrants of C, MLE_dexact.c:323-360): row r < nloc1 is variable 1 at
 // location r, row nloc1 + r variable 2 at location r (columns likewise with
 // nloc2).  One thread per entry, 64 consecutive rows per column.
-__global__ void __launch_bounds__(256) k_gen_bivariate_block(TileView v, BivariateParams P, int first_col) {
+__global__ void __launch_bounds__(256) k_gen_bivariate_block(TileView v, BivariateParams P, long p_off) {
   int m, n, bi, bj;
-  decode_gen_block(v, first_col, m, n, bi, bj);
+  decode_gen_block(v, p_off, m, n, bi, bj);
   const long t = tile_index(v, m, n);
   void* base = v.ptr[t];
   const uint8_t prec = v.prec ? v.prec[t] : (uint8_t)kF64;
@@ -216,9 +217,9 @@ __global__ void __launch_bounds__(256) k_gen_bivariate_block(TileView v, Bivaria
 // (2-D; the time coordinate is not part of it) and u = |t_i - t_j|,
 //   psi = u^(2 alpha) / a + 1,  x = d / (beta_s psi^(beta/2)),
 //   C = sigma^2 M_nu(x) / psi^(beta + delta),   C = sigma^2 / psi^(beta + delta) at d = 0.
-__global__ void __launch_bounds__(256) k_gen_spacetime(TileView v, SpaceTimeParams P, int first_col) {
+__global__ void __launch_bounds__(256) k_gen_spacetime(TileView v, SpaceTimeParams P, long p_off) {
   int m, n, bi, bj;
-  decode_gen_block(v, first_col, m, n, bi, bj);
+  decode_gen_block(v, p_off, m, n, bi, bj);
   const long t = tile_index(v, m, n);
   void* base = v.ptr[t];
   const uint8_t prec = v.prec ? v.prec[t] : (uint8_t)kF64;
@@ -253,31 +254,31 @@ __global__ void __launch_bounds__(256) k_gen_spacetime(TileView v, SpaceTimePara
   }
 }
 
-void launch_gen_spacetime(const TileView& v, const SpaceTimeParams& P, int first_col, long ntiles, hipStream_t s) {
+void launch_gen_spacetime(const TileView& v, const SpaceTimeParams& P, long p_off, long ntiles, hipStream_t s) {
   const int q = v.ts / kNB;
   const long blocks = ntiles * q * q;
   if (blocks <= 0) return;
-  hipLaunchKernelGGL(k_gen_spacetime, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, first_col);
+  hipLaunchKernelGGL(k_gen_spacetime, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, p_off);
   EXA_HIP(hipGetLastError());
 }
 
-void launch_gen_matern(const TileView& v, const MaternParams& P, int first_col, long ntiles, hipStream_t s) {
+void launch_gen_matern(const TileView& v, const MaternParams& P, long p_off, long ntiles, hipStream_t s) {
   const int q = v.ts / kNB;
   const long blocks = ntiles * q * q;
   if (blocks <= 0) return;
-  hipLaunchKernelGGL(k_gen_matern, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, first_col);
+  hipLaunchKernelGGL(k_gen_matern, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, p_off);
   EXA_HIP(hipGetLastError());
 }
 
-void launch_gen_bivariate(const TileView& v, const BivariateParams& P, int first_col, long ntiles, bool block,
+void launch_gen_bivariate(const TileView& v, const BivariateParams& P, long p_off, long ntiles, bool block,
                           hipStream_t s) {
   const int q = v.ts / kNB;
   const long blocks = ntiles * q * q;
   if (blocks <= 0) return;
   if (block)
-    hipLaunchKernelGGL(k_gen_bivariate_block, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, first_col);
+    hipLaunchKernelGGL(k_gen_bivariate_block, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, p_off);
   else
-    hipLaunchKernelGGL(k_gen_bivariate, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, first_col);
+    hipLaunchKernelGGL(k_gen_bivariate, dim3((unsigned)blocks), dim3(64, 4), 0, s, v, P, p_off);
   EXA_HIP(hipGetLastError());
 }
 

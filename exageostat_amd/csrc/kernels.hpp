@@ -55,14 +55,15 @@ struct SpaceTimeParams {
   KnuOrder<double> ko;
 };
 
-// Generation: ntiles tiles starting at tile column first_col (lower
-// matrices: the trailing triangle from first_col; full matrices: all).
-void launch_gen_matern(const TileView& v, const MaternParams& P, int first_col, long ntiles, hipStream_t s);
+// Generation of ntiles tiles starting at tile p_off of the matrix's
+// enumeration (lower matrices: p = m (m + 1) / 2 + n, row by row; full
+// matrices: p = n mt + m).
+void launch_gen_matern(const TileView& v, const MaternParams& P, long p_off, long ntiles, hipStream_t s);
 // block = false: variables interleaved by row parity; true: the rows of
 // variable 1 (P.nloc1 of them), then those of variable 2.
-void launch_gen_bivariate(const TileView& v, const BivariateParams& P, int first_col, long ntiles, bool block,
+void launch_gen_bivariate(const TileView& v, const BivariateParams& P, long p_off, long ntiles, bool block,
                           hipStream_t s);
-void launch_gen_spacetime(const TileView& v, const SpaceTimeParams& P, int first_col, long ntiles, hipStream_t s);
+void launch_gen_spacetime(const TileView& v, const SpaceTimeParams& P, long p_off, long ntiles, hipStream_t s);
 
 // ---- tile Cholesky building blocks (FP64) -------------------------------
 // Factor the diagonal tile (k,k) of a lower-packed matrix in place with
@@ -132,6 +133,11 @@ void launch_axpby(double a, const double* x, double b, const double* y, double* 
 // y = Sigma x for a lower-packed FP64 matrix with full diagonal tiles, each
 // tile read once; part: 2 * mt * (mt * ts) doubles of scratch.
 void launch_symv(const TileView& v, const double* x, double* y, double* part, hipStream_t s);
+// The two halves of launch_symv: the partial products of the tiles p0 ..
+// p0 + ntiles - 1 (row-by-row enumeration p = m (m + 1) / 2 + n), then the
+// sum of every tile's partials into y.
+void launch_symv_tiles(const TileView& v, const double* x, double* part, long p0, long ntiles, hipStream_t s);
+void launch_symv_reduce(const TileView& v, const double* part, double* y, hipStream_t s);
 
 // ---- adjoint gradient ---------------------------------------------------
 struct MaternGradParams {

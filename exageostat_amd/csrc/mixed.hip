@@ -10,9 +10,12 @@
 //     column location sets): strongly correlated tiles stay FP64, weakly
 //     correlated ones become FP32, and with bf16 on the weakest become FP32
 //     storage with BF16 MFMA updates;
-//   * the FP64 Sigma is kept, and the solve Sigma^-1 z is refined in FP64
-//     (x += L~^-T L~^-1 (z - Sigma x)), so z^T Sigma^-1 z reaches FP64
-//     accuracy whatever the off-diagonal precision;
+//   * the solve Sigma^-1 z is refined in FP64 (x += L~^-T L~^-1 (z - Sigma
+//     x)), so z^T Sigma^-1 z reaches FP64 accuracy whatever the off-diagonal
+//     precision.  Sigma x reads the FP64 Sigma, or, in a lean context (no
+//     FP64 Sigma), regenerates Sigma's tiles in FP64 a chunk at a time into
+//     a small pool and multiplies them there: the same values and the same
+//     summation order, so both give the same result;
 //   * log det Sigma comes from the FP64 diagonal tiles of the mixed factor.
 #include <hip/hip_runtime.h>
 
@@ -43,11 +46,24 @@ struct MixedState {
   long bbox_n = 0;
   int bbox_ts = 0;
   int bbox_layout = -1;
+  // Lean contexts: FP64 tiles of Sigma regenerated for Sigma x, `regen_slots`
+  // tiles at a time; regen_ptr maps every lower tile to its slot (tile p of
+  // the row-by-row enumeration to slot p mod regen_slots).
+  double* regen = nullptr;
+  void** regen_ptr = nullptr;
+  long regen_slots = 0;
+  int regen_mt = 0, regen_ts = 0;
   ~MixedState() {
     if (part) (void)hipFree(part);
     if (split) (void)hipFree(split);
+    if (regen) (void)hipFree(regen);
+    if (regen_ptr) (void)hipFree(regen_ptr);
   }
 };
+
+// Tiles regenerated per chunk in a lean context: enough work-groups for the
+// product (one per tile) to fill the GPU; 5.7 GB at ts = 960.
+static constexpr long kRegenSlots = 768;
 
 void destroy_mixed(MixedState* m) { delete m; }
 
@@ -159,13 +175,58 @@ static void symv(const TileMatrix& C, const double* x, double* y, double* part, 
   launch_symv(C.view(), x, y, part, s);
 }
 
+// y = Sigma x without a stored FP64 Sigma: the lower tiles are generated in
+// FP64 into the slot pool a chunk of the row-by-row enumeration at a time,
+// and each chunk's partial products are taken before the next chunk
+// overwrites the slots (one stream, in order).  The generated values, the
+// per-tile partials and their final sum are those of symv() on the stored
+// Sigma.
+static void symv_regen(Context& ctx, MixedState& st, KernelId kid, const double* theta, int metric, const double* x,
+                       double* y, hipStream_t s) {
+  const int mt = ctx.mt, ts = ctx.ts;
+  const long ntiles = (long)mt * (mt + 1) / 2;
+  const long tt = (long)ts * ts;
+  if (!st.regen || st.regen_mt != mt || st.regen_ts != ts) {
+    if (st.regen) (void)hipFree(st.regen);
+    if (st.regen_ptr) (void)hipFree(st.regen_ptr);
+    st.regen = nullptr;
+    st.regen_ptr = nullptr;
+    st.regen_slots = std::min(ntiles, kRegenSlots);
+    EXA_HIP(hipMalloc(&st.regen, sizeof(double) * tt * st.regen_slots));
+    EXA_HIP(hipMalloc(&st.regen_ptr, sizeof(void*) * ntiles));
+    std::vector<void*> h(ntiles);
+    long p = 0;
+    for (int m = 0; m < mt; ++m)
+      for (int n = 0; n <= m; ++n, ++p) {
+        const long t = (long)n * mt - ((long)n * (n - 1)) / 2 + (m - n);  // lower-packed tile index
+        h[t] = st.regen + (p % st.regen_slots) * tt;
+      }
+    EXA_HIP(hipMemcpy(st.regen_ptr, h.data(), sizeof(void*) * ntiles, hipMemcpyHostToDevice));
+    st.regen_mt = mt;
+    st.regen_ts = ts;
+  }
+  TileView v{};
+  v.ptr = st.regen_ptr;
+  v.prec = nullptr;
+  v.m = v.n = ctx.N;
+  v.mt = v.nt = mt;
+  v.ts = ts;
+  v.lower = 1;
+  for (long p0 = 0; p0 < ntiles; p0 += st.regen_slots) {
+    const long cnt = std::min(st.regen_slots, ntiles - p0);
+    generate_covariance_tiles(v, kid, theta, ctx.l1, ctx.l1, metric, p0, cnt, s);
+    launch_symv_tiles(v, x, st.part, p0, cnt, s);
+  }
+  launch_symv_reduce(v, st.part, y, s);
+}
+
 double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric, const MixedOptions& opt,
                     double* logdet_out, double* dotp_out, int* info, MixedReport* rep) {
   if (opt.ir_steps < 0) throw std::invalid_argument("ir_steps must be >= 0");
   if (!(opt.ir_tol >= 0.0)) throw std::invalid_argument("ir_tol must be >= 0");
   if (!ctx.mixed) ctx.mixed = new MixedState();
   MixedState& st = *ctx.mixed;
-  const int N = ctx.N, ts = ctx.ts, mt = ctx.C.mt;
+  const int N = ctx.N, ts = ctx.ts, mt = ctx.mt;
   const long padded = (long)mt * ts;
   hipStream_t s1 = ctx.s1;
 
@@ -197,10 +258,17 @@ double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric,
   st.lists.split[0] = st.split;
   st.lists.split[1] = st.split + 2L * mt * ts * ts;
   st.lists.lolo = opt.lolo;
-  profiled("gen", 0.0, 4.0 * (double)N * (N + 1.0), s1,
-           [&] { generate_covariance(ctx.C, kid, theta, ctx.l1, ctx.l1, metric, s1); });
-  profiled("convert", 0.0, 0.0, s1,
-           [&] { launch_convert_mixed(ctx.C.view(), st.Cm.view(), ctx.C.num_tiles(), s1); });
+  if (ctx.lean) {
+    // straight into the mixed factor's slots, each tile in its precision
+    // (the values the conversion below would give)
+    profiled("gen", 0.0, 4.0 * (double)N * (N + 1.0), s1,
+             [&] { generate_covariance(st.Cm, kid, theta, ctx.l1, ctx.l1, metric, s1); });
+  } else {
+    profiled("gen", 0.0, 4.0 * (double)N * (N + 1.0), s1,
+             [&] { generate_covariance(ctx.C, kid, theta, ctx.l1, ctx.l1, metric, s1); });
+    profiled("convert", 0.0, 0.0, s1,
+             [&] { launch_convert_mixed(ctx.C.view(), st.Cm.view(), ctx.C.num_tiles(), s1); });
+  }
   EXA_HIP(hipStreamSynchronize(s1));
   const double t1 = wall_seconds();
   const int inf = tile_potrf(st.Cm, ctx.ws, s1, ctx.s2, true, &st.lists);
@@ -225,8 +293,12 @@ double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric,
   if (inf == 0) {
     solve(x);
     for (int it = 0; it < opt.ir_steps; ++it) {
-      profiled("symv", 2.0 * (double)N * N, 8.0 * (double)N * (N + 1.0) / 2.0, s1,
-               [&] { symv(ctx.C, x, r, st.part, s1); });
+      if (ctx.lean)
+        profiled("symv_regen", 2.0 * (double)N * N, 0.0, s1,
+                 [&] { symv_regen(ctx, st, kid, theta, metric, x, r, s1); });
+      else
+        profiled("symv", 2.0 * (double)N * N, 8.0 * (double)N * (N + 1.0) / 2.0, s1,
+                 [&] { symv(ctx.C, x, r, st.part, s1); });
       launch_axpby(1.0, z, -1.0, r, r, N, s1);
       solve(r);
       launch_axpby(1.0, x, 1.0, r, x, N, s1);

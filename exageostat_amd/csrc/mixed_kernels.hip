@@ -357,8 +357,9 @@ void launch_update_fused(const TileView& v, int k, const FusedSpans& sp, const u
 //   partT[m][n*ts + j] = sum_i A(m,n)[i][j] x_m[i]   (m > n),
 // with partN = part, partT = part + mt*len, len = mt*ts.  Wave g takes a
 // quarter of the columns, 16 at a time (16 loads in flight per lane); lane
-// r takes rows r, r + 64, ...
-__global__ void __launch_bounds__(256) k_symv_tile(TileView v, const double* x, double* part) {
+// r takes rows r, r + 64, ...  Work-group b takes tile p_off + b of the
+// row-by-row enumeration p = m (m + 1) / 2 + n.
+__global__ void __launch_bounds__(256) k_symv_tile(TileView v, const double* x, double* part, long p_off) {
   __shared__ double xn[1024], xm[1024];
   __shared__ double red[4][1024];
   const int tid = threadIdx.x;
@@ -367,7 +368,7 @@ __global__ void __launch_bounds__(256) k_symv_tile(TileView v, const double* x, 
   const int ts = v.ts;
   const int q = ts / kNB;
   int m, n;
-  tri_decode(blockIdx.x, m, n);
+  tri_decode(p_off + blockIdx.x, m, n);
   const long len = (long)v.mt * ts;
   for (int i = tid; i < ts; i += 256) {
     xn[i] = x[(long)n * ts + i];
@@ -435,13 +436,23 @@ __global__ void k_symv_reduce(const double* part, int mt, int ts, long rows, dou
   y[i] = s;
 }
 
-void launch_symv(const TileView& v, const double* x, double* y, double* part, hipStream_t s) {
+void launch_symv_tiles(const TileView& v, const double* x, double* part, long p0, long ntiles, hipStream_t s) {
   if (!v.lower || v.ts > 1024 || v.ts % kNB) throw std::invalid_argument("symv: lower matrix, ts <= 1024, ts % 64 == 0");
-  const long ntiles = (long)v.mt * (v.mt + 1) / 2;
-  hipLaunchKernelGGL(k_symv_tile, dim3((unsigned)ntiles), dim3(256), 0, s, v, x, part);
+  if (p0 < 0 || ntiles < 0 || p0 + ntiles > (long)v.mt * (v.mt + 1) / 2) throw std::invalid_argument("symv: tile range");
+  if (ntiles == 0) return;
+  hipLaunchKernelGGL(k_symv_tile, dim3((unsigned)ntiles), dim3(256), 0, s, v, x, part, p0);
+  EXA_HIP(hipGetLastError());
+}
+
+void launch_symv_reduce(const TileView& v, const double* part, double* y, hipStream_t s) {
   hipLaunchKernelGGL(k_symv_reduce, dim3((unsigned)((v.m + 255) / 256)), dim3(256), 0, s, part, v.mt, v.ts,
                      (long)v.m, y);
   EXA_HIP(hipGetLastError());
+}
+
+void launch_symv(const TileView& v, const double* x, double* y, double* part, hipStream_t s) {
+  launch_symv_tiles(v, x, part, 0, (long)v.mt * (v.mt + 1) / 2, s);
+  launch_symv_reduce(v, part, y, s);
 }
 
 __global__ void k_axpby(double a, const double* x, double b, const double* y, double* out, long n) {

@@ -104,11 +104,13 @@ class ExactMLE:
 
     t: third coordinate of every location (time for the space-time kernel),
     or None.  computation: "exact" or "diag_approx" (the band-limited
-    factorisation, band = diag_thick tiles)."""
+    factorisation, band = diag_thick tiles).  lean: no FP64 covariance on
+    the device (MLE_data.mp_lean); only precision 1 and 2 likelihoods,
+    prediction and MLOE/MMOM run, and z must come from set_z."""
 
     def __init__(self, x, y, ts: int, kernel: str = "univariate_matern_stationary", dm: str = "ed",
                  precision: int = 0, silent: bool = True, t=None, computation: str = "exact",
-                 diag_thick: int = 1):
+                 diag_thick: int = 1, lean: bool = False):
         self.lib = _lib.load()
         if kernel not in KERNEL_PARAMS:
             raise ValueError(f"unsupported kernel {kernel!r}")
@@ -133,6 +135,7 @@ class ExactMLE:
         d.l1.z = _dp(self.t) if self.t is not None else None
         d.diag_thick = diag_thick
         d.mp_ir_steps = 3
+        d.mp_lean = 1 if lean else 0
         self.data = d
         zero = ctypes.c_int(0)
         one = ctypes.c_int(1)

@@ -303,6 +303,15 @@ typedef struct {
                            -3 (the likelihood is still returned)             */
   int mp_ir_done;       /* mixed, output: refinement steps run               */
   double mp_ir_change;  /* mixed, output: relative change of the last step   */
+  int mp_lean;          /* read by EXAGEOSTAT_dmle_Call: 1 allocates no FP64
+                           covariance.  Only the mixed (precision 2) and
+                           single (precision 1) likelihoods, prediction and
+                           MLOE/MMOM run; the refinement regenerates Sigma's
+                           tiles in FP64 for its products instead of reading
+                           them, so the device holds the mixed factor only
+                           (about half of the FP64 matrix's bytes).  MLE_zvg,
+                           the FP64 likelihood, the gradient, the Fisher
+                           matrix and descC fail with status -1.            */
 } MLE_data;
 
 /* ---------------------------------------------------------------- setup -- */
