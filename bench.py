@@ -287,7 +287,10 @@ def run_c4(steps: int, max_evals: int = 3000, which=C4_FITS) -> None:
     start = np.array([0.5, 0.05, 1.0, 0.05])
     x, y = exageostat_amd.generate_xy_loc(n, 0)
     e, _ = mle.dlarnv(3, [0, 0, 0, 1], n)
-    m = exageostat_amd.ExactMLE(x, y, ts, kernel=kern)
+    # not silent: the library prints the reference's line per evaluation (on
+    # stdout, before the JSON line), so a long derivative-free run shows
+    # progress
+    m = exageostat_amd.ExactMLE(x, y, ts, kernel=kern, silent=False)
     m.zvg(theta, e)
     out = {"config": "C4: n=40000 nugget kernel, L-BFGS-B (adjoint) vs BOBYQA", "theta_true": theta.tolist()}
     m.loglik(start)
@@ -311,7 +314,7 @@ def run_c4(steps: int, max_evals: int = 3000, which=C4_FITS) -> None:
     lb, ub = [0.01] * 4, [5.0] * 4
     fits = {}
     for name, meth, st, tol in [("lbfgsb", "lbfgsb", start, 5), ("bobyqa_ftol1e-5", "bobyqa", None, 5),
-                                ("bobyqa_ftol1e-7", "bobyqa", None, 7)]:
+                                ("bobyqa_ftol1e-7", "bobyqa", None, 7), ("bobyqa_ftol1e-9", "bobyqa", None, 9)]:
         if name not in which:
             continue
         r = optimize.fit(m, lb, ub, start=st, method=meth, opt_tol=tol, max_evals=max_evals)
@@ -320,7 +323,7 @@ def run_c4(steps: int, max_evals: int = 3000, which=C4_FITS) -> None:
         print(f"bench: c4 {name}: {r.wall_s:.1f}s {r.evals} evals theta={np.round(r.theta, 6)} l={r.loglik:.6f}",
               file=sys.stderr, flush=True)
     m.close()
-    for k in ("bobyqa_ftol1e-5", "bobyqa_ftol1e-7"):
+    for k in ("bobyqa_ftol1e-5", "bobyqa_ftol1e-7", "bobyqa_ftol1e-9"):
         if k not in fits or "lbfgsb" not in fits:
             continue
         a = np.array(fits["lbfgsb"]["theta"])
@@ -377,6 +380,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="time the FP64 path only")
     ap.add_argument("--fits", default=",".join(C4_FITS), help="--config c4: comma-separated fits to run")
+    ap.add_argument("--max-evals", type=int, default=3000, help="--config c4: evaluation cap of each fit")
     ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "mloe"],
                     help="c2 (default): the benchmark line; c3, c4, c5: the other configurations' measurements; "
                          "mloe: MLOE/MMOM at n = 90,000")
@@ -402,7 +406,7 @@ def main() -> None:
         run_c3(args.steps)
         return
     if args.config == "c4":
-        run_c4(args.steps, which=args.fits.split(","))
+        run_c4(args.steps, max_evals=args.max_evals, which=args.fits.split(","))
         return
     if args.config == "c5":
         run_c5(args.steps)
