@@ -291,6 +291,7 @@ double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric,
   EXA_HIP(hipMemcpyAsync(x, z, sizeof(double) * padded, hipMemcpyDeviceToDevice, s1));
   MixedReport report;
   if (inf == 0) {
+    double prev_change = 0.0;
     solve(x);
     for (int it = 0; it < opt.ir_steps; ++it) {
       if (ctx.lean)
@@ -305,14 +306,21 @@ double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric,
       ++report.ir_done;
       if (opt.ir_tol > 0.0) {
         // The step changed z^T x by z^T c (c = r, the correction just
-        // added); stop once that is below the tolerance.
+        // added).  Stop once that change is below the tolerance, or once
+        // the change still to come is: the refinement converges linearly,
+        // so with the ratio rho = change / previous change (< 1) the
+        // remaining changes sum to about change * rho / (1 - rho).
         const double dq = device_dot(z, r, N, ctx.d_partial, s1);
         const double q = device_dot(z, x, N, ctx.d_partial, s1);
-        report.ir_change = q != 0.0 ? std::fabs(dq) / std::fabs(q) : std::fabs(dq);
-        if (report.ir_change <= opt.ir_tol) break;
+        const double change = q != 0.0 ? std::fabs(dq) / std::fabs(q) : std::fabs(dq);
+        report.ir_change = change;
+        const double rho = prev_change > 0.0 ? change / prev_change : 1.0;
+        report.ir_converged = change <= opt.ir_tol || (rho < 0.5 && change * rho / (1.0 - rho) <= opt.ir_tol);
+        if (report.ir_converged) break;
+        prev_change = change;
       }
     }
-    report.ir_converged = !(opt.ir_tol > 0.0 && opt.ir_steps > 0 && report.ir_change > opt.ir_tol);
+    if (!(opt.ir_tol > 0.0 && opt.ir_steps > 0)) report.ir_converged = true;
   }
   if (rep) *rep = report;
   const double dotp = device_dot(z, x, N, ctx.d_partial, s1);

@@ -11,7 +11,10 @@ struct MixedOptions {
   double fp32_corr = 0.0;  // ... >= this are FP32, the rest BF16 (when bf16)
   bool bf16 = false;
   int ir_steps = 3;   // at most this many FP64 refinement steps
-  double ir_tol = 0;  // > 0: stop once a step changes z^T x by <= ir_tol (relative)
+  // > 0: stop once a step changes z^T x by <= ir_tol (relative), or once the
+  // changes still to come, extrapolated from the last two (linear
+  // convergence with ratio < 1/2), sum to <= ir_tol
+  double ir_tol = 0;
   int diag_thick = 1;
   bool lolo = false;  // FP32 class: hi*hi + hi*lo + lo*hi (+ lo*lo when set)
 };
@@ -20,7 +23,7 @@ struct MixedOptions {
 struct MixedReport {
   int ir_done = 0;         // refinement steps run
   double ir_change = 0.0;  // |z^T c| / |z^T x| of the last step's correction c (0: no step)
-  bool ir_converged = true;  // false: ir_tol > 0 and the last change is above it
+  bool ir_converged = true;  // false: ir_tol > 0 and the stopping test was never met
 };
 
 double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric, const MixedOptions& opt,

@@ -298,7 +298,10 @@ typedef struct {
                            products of the FP32 tiles (four products instead
                            of three: smaller log-det bias, ~6% slower)       */
   double mp_ir_tol;     /* mixed: > 0 stops the refinement once a step
-                           changes z^T Sigma^-1 z by <= mp_ir_tol (relative);
+                           changes z^T Sigma^-1 z by <= mp_ir_tol (relative),
+                           or once the changes still to come, extrapolated
+                           from the last two steps (linear convergence,
+                           ratio < 1/2), sum to <= mp_ir_tol;
                            if mp_ir_steps steps do not get there, status is
                            -3 (the likelihood is still returned)             */
   int mp_ir_done;       /* mixed, output: refinement steps run               */

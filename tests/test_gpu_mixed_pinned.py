@@ -102,6 +102,29 @@ def test_fused_update_fp64_band_matches_oracle(lib):
     assert np.any(off == 0) and np.any(low == 1)
 
 
+def test_refinement_stopping_rule(lib):
+    """The benchmark's refinement (at most 3 steps, mp_ir_tol = 1e-10) at
+    n = 5,760, ts = 192: it stops as soon as the change of z^T Sigma^-1 z,
+    or the changes extrapolated from the last two steps, are below the
+    tolerance, with the quadratic form still at 1e-9 of the oracle's and
+    status 0; with a tolerance no step can reach, all steps run and the
+    status is -3, the likelihood still returned."""
+    x, y, z, (ll, ld, dp) = _field(5760, THETAS[0])
+    m = lib.ExactMLE(x, y, 192, precision=2)
+    m.set_mixed(**dict(OPTS, ir_tol=1e-10))
+    m.set_z(z)
+    val = m.loglik(THETAS[0])
+    done, change = m.data.mp_ir_done, m.data.mp_ir_change
+    print(f"ir_tol 1e-10: {done} steps, last change {change:.2e}, rel ddotp {abs(m.dotp - dp) / dp:.2e}", flush=True)
+    assert m.status == 0 and 1 <= done <= 3
+    assert abs(m.dotp - dp) <= 1e-9 * abs(dp) and abs(val - ll) <= MIXED_TOL * abs(ll)
+    m.set_mixed(**dict(OPTS, ir_steps=2, ir_tol=1e-300))
+    val = m.loglik(THETAS[0])
+    assert m.status == -3 and m.data.mp_ir_done == 2
+    assert abs(val - ll) <= MIXED_TOL * abs(ll)
+    m.close()
+
+
 def test_mixed_gradient_is_refused(lib):
     """precision 2 with a gradient request: status -1 and an error string,
     never a NaN gradient with status 0."""
