@@ -93,9 +93,11 @@ PEAK_SPLIT_BF16_TFLOPS = PEAK_BF16_TFLOPS / 3.0
 
 # Mixed-precision options of the benchmark: tiles whose largest correlation
 # is below fp32_corr get BF16 MFMA updates, the other off-diagonal tiles are
-# FP32, diagonal tiles FP64; at most 3 FP64 refinement steps on the solve,
-# stopping once a step changes z^T Sigma^-1 z by at most 1e-10 (relative).
-MIXED = dict(fp32_corr=1e-2, bf16=True, fp64_corr=0.0, diag_thick=1, ir_steps=3, ir_tol=1e-10)
+# FP32, diagonal tiles FP64; two FP64 refinement steps on the solve.  At this
+# workload a third step changes z^T Sigma^-1 z by 2e-11 (relative): the
+# log-determinant's error (2.4e-8 relative) is what sets |dl|, and the bench
+# line reports both.
+MIXED = dict(fp32_corr=1e-2, bf16=True, fp64_corr=0.0, diag_thick=1, ir_steps=2, ir_tol=0.0)
 
 
 def _timed(m, theta, steps: int, warmup: int, names):
@@ -180,7 +182,7 @@ def _fp64_vs_mixed(m, theta, steps: int, variants: dict) -> dict:
 # products, 24 bits, measured the same error), so the remaining lever is
 # FP64 storage for the most correlated tiles.  Up to 8 refinement steps (the
 # refinement stops once a step changes z^T Sigma^-1 z by <= 1e-10).
-_M8 = dict(MIXED, ir_steps=8)
+_M8 = dict(MIXED, ir_steps=8, ir_tol=1e-10)
 VARIANTS = {"mixed": _M8, "mixed_lolo": dict(_M8, lolo=True)}
 VARIANTS_C3_ROUGH = dict(VARIANTS, mixed_lolo_fp64corr_0_7=dict(_M8, lolo=True, fp64_corr=0.7))
 VARIANTS_C5 = dict(VARIANTS, mixed_lolo_nobf16_fp64corr_0_3=dict(_M8, lolo=True, bf16=False, fp64_corr=0.3))
@@ -440,6 +442,7 @@ def main() -> None:
         pm = m.prec_map()
         low = pm[np.tril_indices(pm.shape[0])]
         mix["tiles"] = {"f64": int(np.sum(low == 0)), "f32": int(np.sum(low == 1)), "bf16": int(np.sum(low == 3))}
+        mix["ir"] = {"steps_done": int(m.data.mp_ir_done), "last_change": float(m.data.mp_ir_change)}
         print(f"bench: mixed {mix['elapsed'] / args.steps:.3f}s/eval l={mix['ll']:.10f}", file=sys.stderr,
               flush=True)
     m.close()
@@ -508,6 +511,7 @@ def main() -> None:
             # into the mixed factor)
             "ms_per_step_excl_gen": 1000.0 * (mix["elapsed"] / args.steps - mix["phases"]["gen"]),
             "loglik": mix["ll"], "status": mix["status"], "phases_s": mix["phases"], "tiles": mix["tiles"],
+            "refinement": mix["ir"],
             "mixed_options": MIXED, "update_classes": classes,
             "symv_GBps": (_rate(mix["prof"]["symv"], "bytes")[0] or 0) / 1e9,
             "trsv_ms_total": mix["prof"]["trsv"]["ms"] / args.steps,
