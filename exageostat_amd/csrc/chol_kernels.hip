@@ -339,7 +339,9 @@ __global__ void __launch_bounds__(C::NT) k_tile_update(TileView v, int k, int np
 static int g_update_wgs = 0;
 
 // Work-groups of the persistent update: CUs minus a reserve for the panel
-// factorisation (EXAGEOSTAT_RESERVE_CUS, default 16), times the occupancy,
+// factorisation (EXAGEOSTAT_RESERVE_CUS, default 8: at n = 90,000 the FP64
+// factorisation takes 4.40 s with 4 or 8 reserved CUs, 4.52 s with 16 and
+// 4.66 s with 24), times the occupancy,
 // rounded down to a multiple of 8 (XCDs).
 template <class C>
 static int update_grid_cap() {
@@ -349,7 +351,7 @@ static int update_grid_cap() {
   EXA_HIP(hipGetDevice(&dev));
   EXA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   EXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_tile_update<C>, C::NT, 0));
-  int reserve = 16;
+  int reserve = 8;
   if (const char* e = getenv("EXAGEOSTAT_RESERVE_CUS")) reserve = atoi(e);
   if (reserve < 0 || reserve >= cus) reserve = 0;
   cap = (cus - reserve) * (occ > 0 ? occ : 1);
