@@ -282,3 +282,30 @@ def test_predict_full_size_via_likelihood(lib):
         mu = zm[k] + 0.5 * (f[2] - f[0]) * s2
         assert 0.0 < s2 < 1.0
         assert abs(mu - zm[k]) <= 1e-7 * max(1.0, abs(zm[k])), (k, mu, zm[k], s2)
+
+
+# ------------------------------------------------------- single-tile edges
+@pytest.mark.parametrize("n,ts", [(1, 64), (64, 64), (100, 128)])
+@pytest.mark.parametrize("precision", [0, 2])
+def test_single_tile_matches_oracle(lib, n, ts, precision):
+    """One tile row (mt = 1): a single location (Sigma = sigma^2, l in
+    closed form), an exactly full tile and a padded one, on the FP64 and
+    the mixed path (whose only tile is the FP64 diagonal one, so no update
+    runs)."""
+    theta = (1.3, 0.1, 0.5)
+    x, y, _ = datagen.generate_xy_loc(n, 3)
+    e, _ = datagen.dlarnv(3, [3, 3, 3, 1], n)
+    z = likelihood.dzvg(K, x, y, theta, e)
+    ref, ld, dp = likelihood.dmle(K, x, y, z, theta)
+    if n == 1:
+        closed = -0.5 * z[0] ** 2 / theta[0] - 0.5 * np.log(theta[0]) - 0.5 * np.log(2 * np.pi)
+        assert abs(ref - closed) <= 1e-14 * abs(closed)
+    m = lib.ExactMLE(x, y, ts, precision=precision)
+    if precision == 2:
+        m.set_mixed(bf16=True, fp32_corr=0.01, ir_steps=2)
+    m.set_z(z)
+    val = m.loglik(theta)
+    status = m.status
+    m.close()
+    assert status == 0
+    assert abs(val - ref) <= 1e-12 * abs(ref), (val, ref)
