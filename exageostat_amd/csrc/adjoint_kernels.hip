@@ -84,7 +84,7 @@ __device__ __forceinline__ void takahashi_unit(const TileView& v, int k, const d
     if constexpr (HAS_U) {
       // A(r, kk) = Z(m, j)[bi*BM + r, kk]: tile (m, j) of Z for j <= m, Z_jm^T from U for j > m
       const double* A = (j <= m ? tile_f64(v, m, j) : tile_f64(U, j, m)) + bi * C::BM;
-      gemm_nt_acc<C>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
+      gemm_nt_ring<C>(acc, A, ts, B, ts, (int)ts, lds, -1.0);
     } else if (j <= m) {
       gemm_nt_acc<C>(acc, tile_f64(v, m, j) + bi * C::BM, ts, B, ts, (int)ts, lds, -1.0);
     } else {
@@ -101,7 +101,7 @@ __device__ __forceinline__ void takahashi_unit(const TileView& v, int k, const d
 template <class C, bool HAS_U>
 __global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, const double* P, double* W, int nsplit,
                                                          TileView U, long units, int nx) {
-  __shared__ __attribute__((aligned(16))) double lds[C::LDS_DOUBLES];
+  __shared__ __attribute__((aligned(16))) double lds[HAS_U ? RingLds<C>::DOUBLES : C::LDS_DOUBLES];
   const int w = blockIdx.x;
   const int x = w % nx;
   const int gx = gridDim.x / nx;

@@ -83,6 +83,18 @@ __device__ __forceinline__ float2 ld_raw(const float* p) { return ld_global_f2(p
 __device__ __forceinline__ double2 widen(double2 v) { return v; }
 __device__ __forceinline__ double2 widen(float2 v) { return double2{(double)v.x, (double)v.y}; }
 
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(1))) void glb_void_t;
+
+// Waits until at most n vector-memory operations of this wave are pending
+// (expcnt and lgkmcnt left at their maxima).  For the direct-to-LDS copies
+// the counter drops when the data is in LDS.
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
 template <class C>
 __device__ __forceinline__ void zero_acc(dx4 (&acc)[C::FM][C::FN]) {
 #pragma unroll
@@ -200,6 +212,91 @@ __device__ __forceinline__ void gemm_nt_acc(dx4 (&acc)[C::FM][C::FN], const TA* 
     __builtin_amdgcn_sched_barrier(0);
     if (kc + 1 < nk) sstore(buf ^ 1);
     __syncthreads();
+  }
+}
+
+// The same product with the operands copied straight into a three-stage LDS
+// ring by global_load_lds_dwordx4 (no staging registers, no LDS stores from
+// registers), as the BF16 classes do.  Stage layout: A as BK columns of BM
+// doubles (column-major, no padding: each 16-lane quarter of a fragment
+// read takes 16 consecutive doubles), then B likewise; a wave-wide copy
+// fills 1,024 contiguous LDS bytes, lane l taking the two doubles
+// 128 q + 2 l of its operand's stage array, gathered from their (row,
+// column) in the tile.  At step kc each wave waits for its own copies of
+// slice kc, a bare barrier makes every wave's copies visible and ends the
+// reads of slice kc-1, whose buffer then receives slice kc+2.  Plain FP64
+// operands (not K-major); K a multiple of BK; A, B 16-byte aligned, lda and
+// ldb even.
+template <class C>
+struct RingLds {
+  static constexpr int A_ST = C::BK * C::BM, B_ST = C::BK * C::BN;  // doubles per stage
+  static constexpr int STAGE = A_ST + B_ST;
+  static constexpr int DOUBLES = 3 * STAGE;
+  static constexpr int A_COPIES = A_ST / 128, B_COPIES = B_ST / 128;
+  static constexpr int PER_WAVE = (A_COPIES + B_COPIES) / C::NW;
+  static_assert(A_ST % 128 == 0 && B_ST % 128 == 0, "stage arrays must be whole 1 KB copies");
+  static_assert(PER_WAVE * C::NW == A_COPIES + B_COPIES, "copies must split evenly over the waves");
+  static_assert(PER_WAVE < 32, "vmcnt range");
+};
+
+template <class C>
+__device__ __forceinline__ void gemm_nt_ring(dx4 (&acc)[C::FM][C::FN], const double* __restrict__ A, long lda,
+                                             const double* __restrict__ B, long ldb, int K, double* lds,
+                                             double sign) {
+  using L = RingLds<C>;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave / C::WGN;
+  const int wn = wave - wm * C::WGN;
+  const int wave_s = __builtin_amdgcn_readfirstlane(wave);
+  auto issue = [&](int kc, int buf) {
+    const int k0 = kc * C::BK;
+#pragma unroll
+    for (int j = 0; j < L::PER_WAVE; ++j) {
+      int c = wave_s * L::PER_WAVE + j;
+      const bool is_b = c >= L::A_COPIES;
+      if (is_b) c -= L::A_COPIES;
+      const int rows = is_b ? C::BN : C::BM;
+      const int e = c * 128 + 2 * lane;  // element of the operand's stage array
+      const int kk = e / rows;
+      const int r = e - kk * rows;
+      const double* src = is_b ? B + r + (long)(k0 + kk) * ldb : A + r + (long)(k0 + kk) * lda;
+      double* dst = lds + buf * L::STAGE + (is_b ? L::A_ST : 0) + c * 128;
+      __builtin_amdgcn_global_load_lds((glb_void_t*)(uintptr_t)src, (lds_void_t*)dst, 16, 0, 0);
+    }
+  };
+  auto compute = [&](int buf) {
+    const double* Ab = lds + buf * L::STAGE + wm * C::WM + (lane & 15);
+    const double* Bb = lds + buf * L::STAGE + L::A_ST + wn * C::WN + (lane & 15);
+#pragma unroll
+    for (int kk = 0; kk < C::BK; kk += 4) {
+      const int krow = kk + (lane >> 4);
+      double a[C::FM], b[C::FN];
+#pragma unroll
+      for (int i = 0; i < C::FM; ++i) a[i] = sign * Ab[krow * C::BM + i * 16];
+#pragma unroll
+      for (int j = 0; j < C::FN; ++j) b[j] = Bb[krow * C::BN + j * 16];
+#pragma unroll
+      for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+        for (int j = 0; j < C::FN; ++j) acc[i][j] = mfma_f64(a[i], b[j], acc[i][j]);
+    }
+  };
+  const int nk = K / C::BK;
+  if (nk <= 0) return;
+  // The ring may still be read by the previous block's last steps.
+  __builtin_amdgcn_s_barrier();
+  issue(0, 0);
+  if (nk > 1) issue(1, 1);
+  int buf = 0;
+  for (int kc = 0; kc < nk; ++kc) {
+    if (kc + 1 < nk) wait_vmcnt<L::PER_WAVE>();
+    else wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    if (kc + 2 < nk) issue(kc + 2, buf == 0 ? 2 : buf - 1);
+    compute(buf);
+    buf = buf == 2 ? 0 : buf + 1;
   }
 }
 

@@ -132,17 +132,7 @@ struct Bf16dLds {
   static_assert(CHUNKS % C::NW == 0, "copies must split evenly over the waves");
 };
 
-typedef __attribute__((address_space(3))) void lds_void_t;
-typedef __attribute__((address_space(1))) void glb_void_t;
 
-// Waits until at most n vector-memory operations of this wave are pending
-// (expcnt and lgkmcnt left at their maxima).  For the direct-to-LDS copies
-// the counter drops when the data is in LDS.
-template <int N>
-__device__ __forceinline__ void wait_vmcnt() {
-  static_assert(N >= 0 && N < 64, "vmcnt range");
-  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
-}
 
 // acc += A * B^T (A and B rows from the split panels, so the caller
 // subtracts the result), for K a multiple of BK.  The ring holds
