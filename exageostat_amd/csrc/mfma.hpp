@@ -1,10 +1,12 @@
 // FP64 MFMA work-group GEMM core for CDNA4 (gfx950).
 //
 // acc += sign * A * B^T, A: BM x K, B: BN x K, both column-major in global
-// memory.  A and B are staged through LDS in BK-deep slices with double
-// buffering (global loads of slice k+1 are in flight while slice k feeds the
-// matrix cores).  Each 64-lane wave owns a WM x WN sub-block made of 16x16
-// fragments of v_mfma_f64_16x16x4f64.
+// memory.  A and B are staged through LDS in BK-deep slices, either with
+// register-staged double buffering (gemm_nt_acc: global loads of slice k+1
+// are in flight while slice k feeds the matrix cores; FP32 and K-major
+// operands) or with direct-to-LDS copies into a three-stage ring
+// (gemm_nt_ring: plain FP64 operands).  Each 64-lane wave owns a WM x WN
+// sub-block made of 16x16 fragments of v_mfma_f64_16x16x4f64.
 #pragma once
 
 #include "common.hpp"
