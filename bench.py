@@ -265,6 +265,65 @@ def run_c5(steps: int) -> None:
                       "mse_var1": m.data.mserror1, "mse_var2": m.data.mserror2,
                       "mse_check_var1": float(np.mean((za[0::2] - zm[0::2]) ** 2)),
                       "mse_check_var2": float(np.mean((za[1::2] - zm[1::2]) ** 2))}
+    # the same kriging through the mixed path, with the option set that
+    # holds the likelihood to 1e-6 here
+    opts = VARIANTS_C5["mixed_lolo_nobf16_fp64corr_0_3"]
+    m.set_mixed(**opts)
+    t0 = time.perf_counter()
+    zm2, _ = m.predict(theta, xo, yo, zo, xm, ym, za, precision=2)
+    out["kriging"]["mixed"] = {"s": time.perf_counter() - t0, "options": opts, "status": m.status,
+                               "refinement_steps": m.data.mp_ir_done,
+                               "max_rel_diff_vs_fp64": float(np.max(np.abs(zm2 - zm)) / np.max(np.abs(zm))),
+                               "mse_var1": m.data.mserror1, "mse_var2": m.data.mserror2}
+    m.close()
+    print(json.dumps(out), flush=True)
+
+
+def run_kriging(steps: int) -> None:
+    """Kriging at n = 90,000: 1,000 held-out locations drawn at random
+    (seeded) from GenerateXYLoc(90000, 0), 89,000 observed (both sets keep
+    the Morton order), ts = 960, theta = (1, 0.1, 0.5), z = L(theta) e: the
+    FP64 predictor (EXAGEOSTAT_dmle_Predict_Tile) against the mixed-precision
+    one (EXAGEOSTAT_sdmle_Predict_Tile: C22 factored with the benchmark's
+    tile classes, the solve refined in FP64 to 1e-12) and the single-
+    precision one (EXAGEOSTAT_smle_Predict_Tile); one warm-up and `steps`
+    timed calls each, the largest prediction difference against FP64 and the
+    MSEs."""
+    import exageostat_amd
+    from exageostat_amd import mle
+    n, nmiss, ts = 90000, 1000, 960
+    theta = np.array([1.0, 0.1, 0.5])
+    x, y = exageostat_amd.generate_xy_loc(n, 0)
+    e, _ = mle.dlarnv(3, [0, 0, 0, 1], n)
+    m = exageostat_amd.ExactMLE(x, y, ts)
+    m.zvg(theta, e)
+    z = m.get_z()
+    m.close()
+    rng = np.random.default_rng(0)
+    miss = np.sort(rng.choice(n, nmiss, replace=False))
+    obs = np.setdiff1d(np.arange(n), miss)
+    xo, yo, zo, xm, ym, za = x[obs], y[obs], z[obs], x[miss], y[miss], z[miss]
+    m = exageostat_amd.ExactMLE(xo, yo, ts)
+    m.set_mixed(**dict(MIXED, ir_steps=8, ir_tol=1e-12))
+    out = {"config": "kriging: n=90000, 89,000 observed, 1,000 held out, ts=960", "theta": theta.tolist()}
+    ref = None
+    for name, prec in (("fp64", 0), ("mixed", 2), ("single", 1)):
+        m.predict(theta, xo, yo, zo, xm, ym, za, precision=prec)
+        mle.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            zm, mse = m.predict(theta, xo, yo, zo, xm, ym, za, precision=prec)
+        dt = (time.perf_counter() - t0) / steps
+        row = {"s_per_call": dt, "mse": mse, "status": m.status}
+        if ref is None:
+            ref = zm
+        else:
+            row["max_rel_diff_vs_fp64"] = float(np.max(np.abs(zm - ref)) / np.max(np.abs(ref)))
+            row["speedup_vs_fp64"] = out["fp64"]["s_per_call"] / dt
+        if prec == 2:
+            row["refinement_steps"] = m.data.mp_ir_done
+        out[name] = row
+        print(f"bench: kriging {name} {dt:.3f}s mse={mse:.8f}", file=sys.stderr, flush=True)
     m.close()
     print(json.dumps(out), flush=True)
 
@@ -383,9 +442,10 @@ def main() -> None:
     ap.add_argument("--fp64-only", action="store_true", help="time the FP64 path only")
     ap.add_argument("--fits", default=",".join(C4_FITS), help="--config c4: comma-separated fits to run")
     ap.add_argument("--max-evals", type=int, default=3000, help="--config c4: evaluation cap of each fit")
-    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "mloe"],
+    ap.add_argument("--config", default="c2", choices=["c2", "c3", "c4", "c5", "mloe", "kriging"],
                     help="c2 (default): the benchmark line; c3, c4, c5: the other configurations' measurements; "
-                         "mloe: MLOE/MMOM at n = 90,000")
+                         "mloe: MLOE/MMOM at n = 90,000; kriging: FP64, mixed and single-precision kriging at "
+                         "n = 90,000")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy kernel of BYTES (scales the HBM counters) and exit")
     args = ap.parse_args()
@@ -415,6 +475,9 @@ def main() -> None:
         return
     if args.config == "mloe":
         run_mloe(args.steps)
+        return
+    if args.config == "kriging":
+        run_kriging(args.steps)
         return
 
     import exageostat_amd

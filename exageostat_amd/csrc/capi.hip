@@ -361,10 +361,38 @@ void release_prediction_descs(MLE_data* data) {
 // Sigma(lmiss, lobs) generated on the device, C22 factored, alpha =
 // C22^-1 zobs by two triangular solves, zmiss = C12 alpha into ctx->Zmiss.
 // zobs: host values, or null to use ctx->Zobs as the caller filled it.
+// The mixed-precision options of MLE_data (single: every off-diagonal tile
+// FP32, no refinement, as in the FP32 likelihood path).
+MixedOptions mixed_options_of(const MLE_data* data, bool single) {
+  MixedOptions opt;
+  if (single) {
+    opt.ir_steps = 0;
+    return opt;
+  }
+  opt.fp64_corr = data->mp_fp64_corr;
+  opt.fp32_corr = data->mp_fp32_corr;
+  opt.bf16 = data->mp_bf16 != 0;
+  opt.ir_steps = data->mp_ir_steps;
+  opt.ir_tol = data->mp_ir_tol;
+  opt.diag_thick = data->diag_thick;
+  opt.lolo = data->mp_lolo != 0;
+  return opt;
+}
+
+// Kriging Zmiss = C12 C22^-1 Zobs (MLE_dexact.c:688-874).  With mopt, the
+// single- or mixed-precision variant (MLE_sexact.c, MLE_sdexact.c:528-680:
+// the reference factors C22 in its lower precision and solves with that
+// factor): C22 is factored by the mixed-precision engine under mopt (per-
+// theta tile precisions, FP64 refinement of the solve when mopt asks for
+// it) on a context of the observed locations; C12 and the product stay
+// FP64.  The refinement's non-convergence is reported as in the
+// likelihood (status -3, the predictions still returned).
 void predict_core(MLE_data* data, Context* ctx, KernelId kid, const double* theta, int nZmiss, int nZobs,
-                  const location& lmiss, const location& lobs, const double* zobs) {
+                  const location& lmiss, const location& lobs, const double* zobs,
+                  const MixedOptions* mopt = nullptr) {
   const int vars = kernel_vars(kid);
-  if (!ctx->C22.allocated() || ctx->C22.m != nZobs || ctx->C12.m != nZmiss) {
+  if (!ctx->C12.allocated() || ctx->C12.m != nZmiss || ctx->C12.n != nZobs ||
+      (!mopt && (!ctx->C22.allocated() || ctx->C22.m != nZobs))) {
     EXAGEOSTAT_dmle_Predict_Allocate(data, nZmiss / vars, nZobs / vars, ctx->ts, 1, 1, 1);
     if (g_status < 0) throw std::runtime_error(g_error);
   }
@@ -374,13 +402,41 @@ void predict_core(MLE_data* data, Context* ctx, KernelId kid, const double* thet
   ctx->lobs.upload(lobs.x, lobs.y, nobs_loc, s, lobs.z);
   ctx->lmiss.upload(lmiss.x, lmiss.y, nmiss_loc, s, lmiss.z);
   const double t0 = wall_seconds();
-  generate_covariance(ctx->C22, kid, theta, ctx->lobs, ctx->lobs, metric_of(data->dm), s);
-  generate_covariance(ctx->C12, kid, theta, ctx->lmiss, ctx->lobs, metric_of(data->dm), s);
+  const int metric = metric_of(data->dm);
+  generate_covariance(ctx->C12, kid, theta, ctx->lmiss, ctx->lobs, metric, s);
   if (zobs) copy_vec_to_device(ctx->Zobs, zobs, nZobs, s);
-  const int info = tile_potrf(ctx->C22, ctx->ws, s, ctx->s2, true);
-  if (info != 0) throw std::runtime_error("prediction: C22 is not positive definite");
-  tile_trsv(ctx->C22, ctx->ws, ctx->Zobs.d, 0, s);
-  tile_trsv(ctx->C22, ctx->ws, ctx->Zobs.d, 1, s);
+  if (mopt) {
+    if (!ctx->pred || ctx->pred->N != nZobs || ctx->pred->ts != ctx->ts || ctx->pred->lean != ctx->lean) {
+      delete ctx->pred;
+      ctx->pred = nullptr;
+      ctx->pred = new Context(nZobs, ctx->ts, 0, 0, ctx->lean);
+    }
+    Context& po = *ctx->pred;
+    EXA_HIP(hipStreamSynchronize(s));
+    po.l1.upload(lobs.x, lobs.y, nobs_loc, po.s1, lobs.z);
+    EXA_HIP(hipMemcpyAsync(po.Zcpy.d, ctx->Zobs.d, sizeof(double) * nZobs, hipMemcpyDeviceToDevice, po.s1));
+    double logdet = 0.0, dotp = 0.0;
+    int info = 0;
+    MixedReport rep;
+    mixed_loglik(po, kid, theta, metric, *mopt, &logdet, &dotp, &info, &rep);
+    if (info != 0) throw std::runtime_error("prediction: C22 is not positive definite");
+    data->mp_ir_done = rep.ir_done;
+    data->mp_ir_change = rep.ir_change;
+    if (!rep.ir_converged) {
+      set_error(-3, "mixed precision: the iterative refinement of the prediction's solve did not reach mp_ir_tol in "
+                    "mp_ir_steps steps");
+      data->status = -3;
+    }
+    // Zobs then holds C22^-1 Zobs, as after the FP64 path's two solves
+    EXA_HIP(hipMemcpyAsync(ctx->Zobs.d, po.work.d, sizeof(double) * nZobs, hipMemcpyDeviceToDevice, po.s1));
+    EXA_HIP(hipStreamSynchronize(po.s1));
+  } else {
+    generate_covariance(ctx->C22, kid, theta, ctx->lobs, ctx->lobs, metric, s);
+    const int info = tile_potrf(ctx->C22, ctx->ws, s, ctx->s2, true);
+    if (info != 0) throw std::runtime_error("prediction: C22 is not positive definite");
+    tile_trsv(ctx->C22, ctx->ws, ctx->Zobs.d, 0, s);
+    tile_trsv(ctx->C22, ctx->ws, ctx->Zobs.d, 1, s);
+  }
   launch_gemv_full(ctx->C12.view(), ctx->Zobs.d, ctx->Zmiss.d, ctx->gemv_part.d, s);
   EXA_HIP(hipStreamSynchronize(s));
   const double t1 = wall_seconds();
@@ -469,19 +525,7 @@ double evaluate(MLE_data* data, unsigned n, const double* theta, double* grad, P
         if (grad) loglik = adjoint_loglik_grad(*ctx, kid, theta, metric, grad, &logdet, &dotp, &info);
         else loglik = ctx->loglik(kid, theta, metric, &logdet, &dotp, &info, band);
       } else {
-        MixedOptions opt;
-        if (path == Path::kMixed) {
-          opt.fp64_corr = data->mp_fp64_corr;
-          opt.fp32_corr = data->mp_fp32_corr;
-          opt.bf16 = data->mp_bf16 != 0;
-          opt.ir_steps = data->mp_ir_steps;
-          opt.ir_tol = data->mp_ir_tol;
-          opt.diag_thick = data->diag_thick;
-          opt.lolo = data->mp_lolo != 0;
-        } else {
-          // single precision: every off-diagonal tile FP32, no refinement
-          opt.ir_steps = 0;
-        }
+        const MixedOptions opt = mixed_options_of(data, path == Path::kSingle);
         loglik = mixed_loglik(*ctx, kid, theta, metric, opt, &logdet, &dotp, &info, &rep);
         data->mp_ir_done = rep.ir_done;
         data->mp_ir_change = rep.ir_change;
@@ -1176,9 +1220,10 @@ void prediction_finalize(MLE_data* data) {
   }, 0);
 }
 
-double EXAGEOSTAT_dmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
-                                    double* Zactual, double* Zmiss, int n) {
-  (void)n;
+// The prediction entry points of the FP64, single- and mixed-precision
+// paths (mopt null: FP64).
+static double predict_entry(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs, double* Zactual,
+                            double* Zmiss, const MixedOptions* mopt) {
   if (nZmiss <= 0) {
     fprintf(stderr, " Number of missing values should be positive value\n");
     return -1;
@@ -1188,7 +1233,8 @@ double EXAGEOSTAT_dmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, i
     const KernelId kid = kernel_from_name(data->kernel_fun);
     if (kid == kUnknownKernel) throw std::invalid_argument("unsupported kernel");
     const int vars = kernel_vars(kid);
-    predict_core(data, ctx, kid, theta, nZmiss, nZobs, data->lmiss, data->lobs, Zobs);
+    data->status = 0;
+    predict_core(data, ctx, kid, theta, nZmiss, nZobs, data->lmiss, data->lobs, Zobs, mopt);
     hipStream_t s = ctx->s1;
     EXA_HIP(hipMemcpyAsync(Zmiss, ctx->Zmiss.d, sizeof(double) * nZmiss, hipMemcpyDeviceToHost, s));
     EXA_HIP(hipStreamSynchronize(s));
@@ -1200,24 +1246,36 @@ double EXAGEOSTAT_dmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, i
   }, -1.0);
 }
 
+double EXAGEOSTAT_dmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
+                                    double* Zactual, double* Zmiss, int n) {
+  (void)n;
+  return predict_entry(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, nullptr);
+}
+
 double MORSE_dmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
                                double* Zactual, double* Zmiss, int n) {
   return EXAGEOSTAT_dmle_Predict_Tile(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, n);
 }
 
-// The prediction of the single, mixed and band-limited paths is the FP64
-// predictor (MLE_sexact.c, MLE_sdexact.c and MLE_approx.c each carry a
-// prediction of their own; here one FP64 kriging serves all).
+// Single precision (MLE_sexact.c's prediction): C22 factored with every
+// off-diagonal tile FP32 (diagonal tiles FP64), solved with that factor.
 double EXAGEOSTAT_smle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
                                     double* Zactual, double* Zmiss, int n) {
-  return EXAGEOSTAT_dmle_Predict_Tile(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, n);
+  (void)n;
+  const MixedOptions opt = mixed_options_of(data, true);
+  return predict_entry(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, &opt);
 }
 
+// Mixed precision (MLE_sdexact.c:528-680): C22 factored under the mp_*
+// options of the mixed likelihood, the solve refined in FP64.
 double EXAGEOSTAT_sdmle_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
                                      double* Zactual, double* Zmiss, int n) {
-  return EXAGEOSTAT_dmle_Predict_Tile(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, n);
+  (void)n;
+  const MixedOptions opt = mixed_options_of(data, false);
+  return predict_entry(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, &opt);
 }
 
+// The band-limited path's prediction (MLE_approx.c) is the FP64 kriging here.
 double EXAGEOSTAT_dmle_diag_Predict_Tile(MLE_data* data, double* theta, int nZmiss, int nZobs, double* Zobs,
                                          double* Zactual, double* Zmiss, int n) {
   return EXAGEOSTAT_dmle_Predict_Tile(data, theta, nZmiss, nZobs, Zobs, Zactual, Zmiss, n);

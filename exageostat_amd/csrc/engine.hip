@@ -691,6 +691,8 @@ TileMatrix& Context::fp64(const char* what) {
 }
 
 Context::~Context() {
+  delete pred;
+  pred = nullptr;
   destroy_mixed(mixed);
   destroy_adjoint(adjoint);
   destroy_mloe(mloe);

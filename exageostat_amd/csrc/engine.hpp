@@ -226,6 +226,10 @@ class Context {
   AdjointState* adjoint = nullptr;  // gradient buffers, created on first use
   MloeState* mloe = nullptr;        // MLOE / MMOM buffers, created on first use
   ClassLists band_lists;            // update pairs of the band-limited factorisation
+  // The observed locations as a context of their own, for the kriging of
+  // the single- and mixed-precision paths (their factor of C22 and the
+  // refined C22^-1 z_obs); created on first use.
+  Context* pred = nullptr;
 
   // band > 0: the band-limited factorisation (computation "diag_approx");
   // the log-determinant and the solve then read the whole lower triangle,

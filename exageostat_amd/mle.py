@@ -262,9 +262,13 @@ class ExactMLE:
         return F, Finv
 
     # -- prediction ------------------------------------------------------
-    def predict(self, theta, xobs, yobs, zobs, xmiss, ymiss, zactual=None, tobs=None, tmiss=None):
-        """EXAGEOSTAT_dmle_Predict_Tile; returns (zmiss, mse).  tobs/tmiss:
-        third coordinates (time) of the observed and held-out locations."""
+    def predict(self, theta, xobs, yobs, zobs, xmiss, ymiss, zactual=None, tobs=None, tmiss=None, precision=None):
+        """Kriging; returns (zmiss, mse).  tobs/tmiss: third coordinates
+        (time) of the observed and held-out locations.  precision (default:
+        the model's): 0 EXAGEOSTAT_dmle_Predict_Tile (FP64), 1
+        EXAGEOSTAT_smle_Predict_Tile (C22 factored with FP32 off-diagonal
+        tiles), 2 EXAGEOSTAT_sdmle_Predict_Tile (C22 factored under the
+        set_mixed options, the solve refined in FP64)."""
         xo = np.ascontiguousarray(xobs, dtype=np.float64)
         yo = np.ascontiguousarray(yobs, dtype=np.float64)
         xm = np.ascontiguousarray(xmiss, dtype=np.float64)
@@ -284,9 +288,15 @@ class ExactMLE:
         th = np.ascontiguousarray(theta, dtype=np.float64)
         self.lib.EXAGEOSTAT_dmle_Predict_Allocate(ctypes.byref(d), len(xm), len(xo), self.ts, 1, 1, 1)
         _check(self.lib, "Predict_Allocate")
-        mse = self.lib.EXAGEOSTAT_dmle_Predict_Tile(ctypes.byref(d), _dp(th), n_miss, n_obs, _dp(zo),
-                                                    _dp(za) if za is not None else None, _dp(zm), self.N)
-        _check(self.lib, "Predict_Tile")
+        prec = self.precision if precision is None else precision
+        name = {0: "EXAGEOSTAT_dmle_Predict_Tile", 1: "EXAGEOSTAT_smle_Predict_Tile",
+                2: "EXAGEOSTAT_sdmle_Predict_Tile"}[prec]
+        mse = getattr(self.lib, name)(ctypes.byref(d), _dp(th), n_miss, n_obs, _dp(zo),
+                                      _dp(za) if za is not None else None, _dp(zm), self.N)
+        if self.lib.exageostat_gpu_last_status() != -3:
+            # -3: the mixed refinement of the solve did not reach mp_ir_tol;
+            # the predictions are still returned and self.status says so
+            _check(self.lib, name)
         return zm, mse
 
     def mloe_mmom(self, theta_true, theta_est, xobs, yobs, xmiss, ymiss):

@@ -428,8 +428,12 @@ double MORSE_dmle_Predict_Tile(MLE_data *data, double *theta, int nZmiss, int nZ
  * the caller with EXAGEOSTAT_Lapack_to_Tile), predictions into
  * data->descZmiss; returns the MSE (-1 without actual values). */
 double EXAGEOSTAT_dmle_Predict_Tile_Async(MLE_data *data, double *theta, int nZmiss, int nZobs, int n);
-/* MLE_exact_s.h:47-50, MLE_sdexact.h:51-54, MLE_approx.h:47-50: the FP64
- * predictor serves every path here. */
+/* MLE_exact_s.h:47-50 (single precision: C22 factored with FP32 off-diagonal
+ * tiles, solved with that factor), MLE_sdexact.h:51-54 (mixed precision: C22
+ * factored under the mp_* options, the solve refined in FP64; status -3
+ * when the refinement does not reach mp_ir_tol, the predictions still
+ * returned), MLE_approx.h:47-50 (the FP64 predictor).  C12 and the product
+ * stay FP64 in all three. */
 double EXAGEOSTAT_smle_Predict_Tile(MLE_data *data, double *theta, int nZmiss, int nZobs, double *Zobs,
                                     double *Zactual, double *Zmiss, int n);
 double EXAGEOSTAT_sdmle_Predict_Tile(MLE_data *data, double *theta, int nZmiss, int nZobs, double *Zobs,
