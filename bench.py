@@ -328,7 +328,10 @@ def run_kriging(steps: int) -> None:
     print(json.dumps(out), flush=True)
 
 
-C4_FITS = ("lbfgsb", "bobyqa_ftol1e-5", "bobyqa_ftol1e-7")
+C4_FITS = ("lbfgsb", "lbfgsb_mixed", "bobyqa_ftol1e-5", "bobyqa_ftol1e-7")
+# the mixed path's options for the gradient-based fit: the benchmark's tile
+# classes, the solve refined to 1e-12
+C4_MIXED = dict(MIXED, ir_steps=8, ir_tol=1e-12)
 
 
 def run_c4(steps: int, max_evals: int = 3000, which=C4_FITS) -> None:
@@ -372,19 +375,41 @@ def run_c4(steps: int, max_evals: int = 3000, which=C4_FITS) -> None:
         fd[p] = (m.loglik(tp) - m.loglik(tm)) / (2 * h)
     out["grad_check_max_rel"] = float(np.max(np.abs(g0 - fd)) / np.max(np.abs(fd)))
     print(f"bench: c4 l {out['loglik_s']:.3f}s, l+grad {out['loglik_grad_s']:.3f}s", file=sys.stderr, flush=True)
+    # the same at precision 2: l from the mixed path, the gradient from its
+    # refined solve and the Takahashi sweep of its factor
+    m.precision = 2
+    m.set_mixed(**C4_MIXED)
+    m.loglik(start, grad=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lm, gm = m.loglik(start, grad=True)
+    mixed = {"loglik_grad_s": (time.perf_counter() - t0) / steps, "options": C4_MIXED, "status": m.status,
+             "rel_dl_vs_fp64": abs(lm - l0) / abs(l0),
+             "grad_max_rel_diff_vs_fp64": float(np.max(np.abs(gm - g0)) / np.max(np.abs(g0)))}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.loglik(start)
+    mixed["loglik_s"] = (time.perf_counter() - t0) / steps
+    out["mixed"] = mixed
+    m.precision = 0
+    print(f"bench: c4 mixed l {mixed['loglik_s']:.3f}s, l+grad {mixed['loglik_grad_s']:.3f}s", file=sys.stderr,
+          flush=True)
     lb, ub = [0.01] * 4, [5.0] * 4
     fits = {}
-    for name, meth, st, tol in [("lbfgsb", "lbfgsb", start, 5), ("bobyqa_ftol1e-5", "bobyqa", None, 5),
-                                ("bobyqa_ftol1e-7", "bobyqa", None, 7), ("bobyqa_ftol1e-9", "bobyqa", None, 9)]:
+    for name, meth, st, tol in [("lbfgsb", "lbfgsb", start, 5), ("lbfgsb_mixed", "lbfgsb", start, 5),
+                                ("bobyqa_ftol1e-5", "bobyqa", None, 5), ("bobyqa_ftol1e-7", "bobyqa", None, 7),
+                                ("bobyqa_ftol1e-9", "bobyqa", None, 9)]:
         if name not in which:
             continue
+        m.precision = 2 if name.endswith("_mixed") else 0
         r = optimize.fit(m, lb, ub, start=st, method=meth, opt_tol=tol, max_evals=max_evals)
+        m.precision = 0
         fits[name] = {"theta": r.theta.tolist(), "loglik": r.loglik, "evals": r.evals, "wall_s": r.wall_s,
                       "message": r.message}
         print(f"bench: c4 {name}: {r.wall_s:.1f}s {r.evals} evals theta={np.round(r.theta, 6)} l={r.loglik:.6f}",
               file=sys.stderr, flush=True)
     m.close()
-    for k in ("bobyqa_ftol1e-5", "bobyqa_ftol1e-7", "bobyqa_ftol1e-9"):
+    for k in ("lbfgsb_mixed", "bobyqa_ftol1e-5", "bobyqa_ftol1e-7", "bobyqa_ftol1e-9"):
         if k not in fits or "lbfgsb" not in fits:
             continue
         a = np.array(fits["lbfgsb"]["theta"])

@@ -144,32 +144,22 @@ static void reduce_sums(AdjointState& st, int nsum, hipStream_t s, double* out) 
   }
 }
 
-double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric, double* grad,
-                           double* logdet_out, double* dotp_out, int* info) {
-  if (kid != kUnivariate && kid != kNugget && kid != kBivariate) throw std::invalid_argument("unsupported kernel");
+// The gradient from a factor of Sigma held in FP64 in A (ws.factored ==
+// A): alpha = Sigma^-1 z (ctx.work holds L^-1 z, or with work_is_alpha
+// Sigma^-1 z itself), Z = Sigma^-1 in place of the factor by the Takahashi
+// sweep, then the fused dSigma reduction.
+static void gradient_from_factor(Context& ctx, KernelId kid, const double* theta, int metric, TileMatrix& A,
+                                 bool work_is_alpha, double* grad) {
   const int np = kernel_num_params(kid);
-  const double t0 = wall_seconds();
-  int inf = 0;
-  double logdet = 0.0, dotp = 0.0;
-  const double ll = ctx.loglik(kid, theta, metric, &logdet, &dotp, &inf);
-  PhaseTimes lt = ctx.last;
-  if (info) *info = inf;
-  if (logdet_out) *logdet_out = logdet;
-  if (dotp_out) *dotp_out = dotp;
-  if (inf != 0) {
-    for (int p = 0; p < np; ++p) grad[p] = std::nan("");
-    return ll;
-  }
   AdjointState& st = adjoint_buffers(ctx);
-  TileMatrix& A = ctx.fp64("the gradient and the Fisher information");
   const TileView v = A.view();
   hipStream_t s = ctx.s1;
 
-  // alpha = L^-T (L^-1 z); ctx.work holds L^-1 z after loglik.
   st.alpha.allocate(ctx.N, ctx.work.padded);
   EXA_HIP(hipMemcpyAsync(st.alpha.d, ctx.work.d, sizeof(double) * ctx.work.padded, hipMemcpyDeviceToDevice, s));
-  profiled("trsv", (double)ctx.N * (ctx.N + 1.0), 4.0 * (double)ctx.N * (ctx.N + 1.0), s,
-           [&] { tile_trsv(A, ctx.ws, st.alpha.d, 1, s); });
+  if (!work_is_alpha)  // alpha = L^-T (L^-1 z)
+    profiled("trsv", (double)ctx.N * (ctx.N + 1.0), 4.0 * (double)ctx.N * (ctx.N + 1.0), s,
+             [&] { tile_trsv(A, ctx.ws, st.alpha.d, 1, s); });
   tile_potri_inplace(A, ctx.ws, st.P, st.Q, s, st.W, kPotriSplit, &st.U);
 
   const int nsum = kid == kBivariate ? 6 : 4;
@@ -209,6 +199,55 @@ double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int 
   reduce_sums(st, nsum, s, acc);
   if (Profiler::get().enabled()) Profiler::get().collect();
   for (int p = 0; p < np; ++p) grad[p] = -0.5 * acc[p];
+}
+
+double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric, double* grad,
+                           double* logdet_out, double* dotp_out, int* info) {
+  if (kid != kUnivariate && kid != kNugget && kid != kBivariate) throw std::invalid_argument("unsupported kernel");
+  const int np = kernel_num_params(kid);
+  const double t0 = wall_seconds();
+  int inf = 0;
+  double logdet = 0.0, dotp = 0.0;
+  const double ll = ctx.loglik(kid, theta, metric, &logdet, &dotp, &inf);
+  PhaseTimes lt = ctx.last;
+  if (info) *info = inf;
+  if (logdet_out) *logdet_out = logdet;
+  if (dotp_out) *dotp_out = dotp;
+  if (inf != 0) {
+    for (int p = 0; p < np; ++p) grad[p] = std::nan("");
+    return ll;
+  }
+  gradient_from_factor(ctx, kid, theta, metric, ctx.fp64("the gradient and the Fisher information"), false, grad);
+  const double t1 = wall_seconds();
+  ctx.last = lt;
+  ctx.last.grad = t1 - t0 - lt.total;
+  ctx.last.total = t1 - t0;
+  return ll;
+}
+
+double mixed_adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric,
+                                 const MixedOptions& opt, double* grad, double* logdet_out, double* dotp_out,
+                                 int* info, MixedReport* rep) {
+  if (kid != kUnivariate && kid != kNugget && kid != kBivariate) throw std::invalid_argument("unsupported kernel");
+  const int np = kernel_num_params(kid);
+  TileMatrix& A = ctx.fp64("the mixed-precision gradient");
+  const double t0 = wall_seconds();
+  int inf = 0;
+  double logdet = 0.0, dotp = 0.0;
+  const double ll = mixed_loglik(ctx, kid, theta, metric, opt, &logdet, &dotp, &inf, rep);
+  PhaseTimes lt = ctx.last;
+  if (info) *info = inf;
+  if (logdet_out) *logdet_out = logdet;
+  if (dotp_out) *dotp_out = dotp;
+  if (inf != 0) {
+    for (int p = 0; p < np; ++p) grad[p] = std::nan("");
+    return ll;
+  }
+  // The FP64 Sigma has served the refinement; it now receives the mixed
+  // factor, whose Takahashi sweep gives the trace term.  ctx.work holds the
+  // refined Sigma^-1 z.
+  mixed_factor_to_fp64(ctx, A, ctx.s1);
+  gradient_from_factor(ctx, kid, theta, metric, A, true, grad);
   const double t1 = wall_seconds();
   ctx.last = lt;
   ctx.last.grad = t1 - t0 - lt.total;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "engine.hpp"
+#include "mixed.hpp"
 
 namespace exa {
 
@@ -21,6 +22,13 @@ void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, 
 
 double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric, double* grad,
                            double* logdet_out, double* dotp_out, int* info);
+// l from the mixed-precision path and its gradient with alpha = the refined
+// Sigma^-1 z and the trace term from the Takahashi sweep of the mixed factor
+// (converted to FP64 in place of the FP64 Sigma, which a lean context does
+// not hold).
+double mixed_adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric,
+                                 const MixedOptions& opt, double* grad, double* logdet_out, double* dotp_out,
+                                 int* info, MixedReport* rep);
 
 // Fisher information F_jk = 1/2 tr(Sigma^-1 dSigma_j Sigma^-1 dSigma_k) of
 // the univariate Matern kernels at theta (np x np, row-major, symmetric).

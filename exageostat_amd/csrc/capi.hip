@@ -491,13 +491,14 @@ double evaluate(MLE_data* data, unsigned n, const double* theta, double* grad, P
     const int metric = metric_of(data->dm);
     const double* lz = (kid == kSpaceTime || data->l1.z) ? data->l1.z : nullptr;
     if (grad) {
-      // The mixed and single factors carry no Sigma^-1 accurate enough for
-      // the trace term; a gradient-based optimiser must not iterate on a
-      // made-up value, so those calls fail and say why.
-      if (path != Path::kExact)
+      // The single-precision factor (no refinement) and the band-limited one
+      // carry no Sigma^-1 accurate enough for the trace term; a gradient-
+      // based optimiser must not iterate on a made-up value, so those calls
+      // fail and say why.
+      if (path != Path::kExact && path != Path::kMixed)
         throw std::invalid_argument(
-            "only precision 0 with computation \"exact\" computes a gradient: pass grad = NULL for precision "
-            "1 or 2 and for diag_approx");
+            "only precision 0 with computation \"exact\" and precision 2 compute a gradient: pass grad = NULL "
+            "for precision 1 and for diag_approx");
       if (!(kid == kUnivariate || kid == kNugget || kid == kBivariate) || profile || lz)
         throw std::invalid_argument(
             "the adjoint gradient covers the 2-D univariate Matern, nugget and interleaved parsimonious bivariate "
@@ -526,7 +527,8 @@ double evaluate(MLE_data* data, unsigned n, const double* theta, double* grad, P
         else loglik = ctx->loglik(kid, theta, metric, &logdet, &dotp, &info, band);
       } else {
         const MixedOptions opt = mixed_options_of(data, path == Path::kSingle);
-        loglik = mixed_loglik(*ctx, kid, theta, metric, opt, &logdet, &dotp, &info, &rep);
+        if (grad) loglik = mixed_adjoint_loglik_grad(*ctx, kid, theta, metric, opt, grad, &logdet, &dotp, &info, &rep);
+        else loglik = mixed_loglik(*ctx, kid, theta, metric, opt, &logdet, &dotp, &info, &rep);
         data->mp_ir_done = rep.ir_done;
         data->mp_ir_change = rep.ir_change;
         if (path == Path::kSingle) data->sdotp = (float)dotp;

@@ -108,6 +108,8 @@ void launch_zero_upper_diag(const TileView& v, hipStream_t s);
 // ---- mixed precision ---------------------------------------------------
 // dst <- src tile by tile (dst tiles FP64 or FP32 by dst.prec).
 void launch_convert_mixed(const TileView& src, const TileView& dst, long ntiles, hipStream_t s);
+// dst (FP64) <- src (mixed precision, same tiling): every tile widened.
+void launch_mixed_to_fp64(const TileView& src, const TileView& dst, long ntiles, hipStream_t s);
 // BF16 hi/lo parts of the panel tiles (m, k), m > k, for the BF16 update
 // classes: tile m at split + 2 m ts^2 (mt tiles of room).
 void launch_split_panel(const TileView& v, int k, uint16_t* split, hipStream_t s);

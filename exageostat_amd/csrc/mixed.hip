@@ -339,6 +339,16 @@ double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric,
   return -0.5 * dotp - 0.5 * logdet - (N / 2.0) * std::log(2.0 * M_PI);
 }
 
+void mixed_factor_to_fp64(Context& ctx, TileMatrix& dst, hipStream_t s) {
+  if (!ctx.mixed || !ctx.mixed->Cm.allocated() || ctx.ws.factored != ctx.mixed->Cm.tile_raw(0, 0))
+    throw std::invalid_argument("no mixed-precision factor to widen");
+  const TileMatrix& src = ctx.mixed->Cm;
+  if (!dst.allocated() || dst.mt != src.mt || dst.ts != src.ts || !dst.lower)
+    throw std::invalid_argument("the FP64 matrix must have the mixed factor's tile grid");
+  launch_mixed_to_fp64(src.view(), dst.view(), src.num_tiles(), s);
+  ctx.ws.factored = dst.tile_raw(0, 0);
+}
+
 // Tile precision map of the last mixed evaluation (lower-packed order);
 // returns the number of tiles or 0 when no mixed factor exists.
 long mixed_prec_map(const Context& ctx, uint8_t* out, long cap) {

@@ -28,6 +28,10 @@ struct MixedReport {
 
 double mixed_loglik(Context& ctx, KernelId kid, const double* theta, int metric, const MixedOptions& opt,
                     double* logdet_out, double* dotp_out, int* info, MixedReport* rep = nullptr);
+// dst (FP64, the mixed factor's tile grid) <- the mixed factor of the last
+// mixed evaluation, every tile widened to FP64; ws then names dst as the
+// factored matrix (its inverted diagonal tiles are the mixed factor's).
+void mixed_factor_to_fp64(Context& ctx, TileMatrix& dst, hipStream_t s);
 // Copies the tile precision map (TilePrec, lower-packed tile order) of the
 // last mixed evaluation into out[0:cap); returns the tile count (0: none).
 long mixed_prec_map(const Context& ctx, uint8_t* out, long cap);

@@ -49,6 +49,33 @@ void launch_convert_mixed(const TileView& src, const TileView& dst, long ntiles,
   EXA_HIP(hipGetLastError());
 }
 
+// dst (FP64) <- src (mixed, same tiling): the reverse of k_convert_mixed.
+__global__ void __launch_bounds__(256) k_mixed_to_fp64(TileView src, TileView dst) {
+  const long t = blockIdx.x >> 2;
+  const int part = blockIdx.x & 3;
+  const long nel = (long)src.ts * src.ts;
+  const long beg = part * (nel / 4), end = beg + nel / 4;
+  double* d = static_cast<double*>(dst.ptr[t]);
+  const uint8_t p = src.prec ? src.prec[t] : (uint8_t)kF64;
+  if (p == kF64) {
+    const double* s = static_cast<const double*>(src.ptr[t]);
+    for (long i = beg + 2 * threadIdx.x; i < end; i += 512)
+      *reinterpret_cast<double2*>(d + i) = *reinterpret_cast<const double2*>(s + i);
+  } else {
+    const float* s = static_cast<const float*>(src.ptr[t]);
+    for (long i = beg + 2 * threadIdx.x; i < end; i += 512) {
+      const float2 v = *reinterpret_cast<const float2*>(s + i);
+      *reinterpret_cast<double2*>(d + i) = double2{(double)v.x, (double)v.y};
+    }
+  }
+}
+
+void launch_mixed_to_fp64(const TileView& src, const TileView& dst, long ntiles, hipStream_t s) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(k_mixed_to_fp64, dim3((unsigned)(ntiles * 4)), dim3(256), 0, s, src, dst);
+  EXA_HIP(hipGetLastError());
+}
+
 // ---------------------------------------------------------- panel split --
 // Panel tiles (m, k), m in (k, mt), converted once per step into BF16 hi and
 // lo parts for the BF16 classes: tile m of `split` holds hi at [0, ts^2) and

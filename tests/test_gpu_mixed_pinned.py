@@ -8,8 +8,8 @@ dense FP64 likelihood: ts = 192 and 384 (fused launch), ts = 1,024 (the
 tile size of the n = 160,000 workload, per-class launches), at the
 benchmark's theta and at the two n = 160,000 thetas, with every tile class
 present.  They also pin kriging with the parsimonious bivariate kernel
-against the oracle's predictor, and the mixed path's refusal to return a
-gradient.
+against the oracle's predictor, the mixed path's gradient against the
+oracle's, and the single-precision path's refusal to return one.
 """
 import numpy as np
 import pytest
@@ -20,6 +20,7 @@ pytestmark = pytest.mark.gpu
 
 K = "univariate_matern_stationary"
 MIXED_TOL = 1e-6
+GRAD_TOL = 1e-4
 THETAS = [(1.0, 0.1, 0.5), (1.0, 0.03, 1.0), (1.0, 0.3, 0.5)]
 # the benchmark's option set: FP32 tiles, BF16 updates below correlation 0.01
 OPTS = dict(bf16=True, fp32_corr=0.01, ir_steps=3)
@@ -125,15 +126,38 @@ def test_refinement_stopping_rule(lib):
     m.close()
 
 
-def test_mixed_gradient_is_refused(lib):
-    """precision 2 with a gradient request: status -1 and an error string,
+@pytest.mark.parametrize("n,ts", [(1600, 320), (5760, 192)], ids=["per_class", "fused"])
+def test_mixed_gradient_matches_oracle(lib, n, ts):
+    """precision 2 with a gradient request: alpha from the refined solve,
+    the trace term from the Takahashi sweep of the mixed factor; every
+    component against the oracle's dense FP64 gradient, relative to the
+    gradient's largest component."""
+    x, y, z, (ll, _, _) = _field(5760, THETAS[0])
+    x, y, z = x[:n], y[:n], z[:n]
+    l_ref, g_ref = likelihood.gradient(K, x, y, z, np.array(THETAS[0]))
+    m = lib.ExactMLE(x, y, ts, precision=2)
+    m.set_mixed(**dict(OPTS, ir_steps=8, ir_tol=1e-12))
+    m.set_z(z)
+    val, g = m.loglik(np.array(THETAS[0]), grad=True)
+    status = m.status
+    # the context keeps working after the in-place inverse
+    val2 = m.loglik(np.array(THETAS[0]))
+    m.close()
+    err = np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref))
+    print(f"mixed gradient n={n} ts={ts}: rel err {err:.2e}, g={g}, oracle={g_ref}", flush=True)
+    assert status == 0
+    assert abs(val - l_ref) <= MIXED_TOL * abs(l_ref) and abs(val2 - l_ref) <= MIXED_TOL * abs(l_ref)
+    assert err <= GRAD_TOL, err
+
+
+def test_single_precision_gradient_is_refused(lib):
+    """precision 1 with a gradient request: status -1 and an error string,
     never a NaN gradient with status 0."""
     import ctypes
 
     from exageostat_amd import _lib
     x, y, z, _ = _field(5760, THETAS[0])
-    m = lib.ExactMLE(x[:1600], y[:1600], 320, precision=2)
-    m.set_mixed(**OPTS)
+    m = lib.ExactMLE(x[:1600], y[:1600], 320, precision=1)
     m.set_z(z[:1600])
     th = np.array(THETAS[0])
     g = np.zeros(3)
