@@ -364,8 +364,11 @@ void EXAGEOSTAT_MLE_dzvg_Tile_Async(MLE_data *data, double *Nrand, double *initi
  * (EXAGEOSTAT_sdmle_Tile); "diag_approx" with precision 0
  * (EXAGEOSTAT_dmle_diag_Tile).  Other combinations return -HUGE_VAL with
  * status -1 (the reference returns 0).  grad, when non-NULL, receives
- * d l / d theta on the precision-0 exact path (the reference ignores it);
- * the other paths then fail with status -1.  The "_profile" bivariate
+ * d l / d theta on the precision-0 exact path and on the precision-2
+ * mixed path (the reference ignores it; the mixed gradient takes alpha
+ * from the refined solve and the trace term from the mixed factor, and
+ * needs a context that holds the FP64 covariance); precision 1 and
+ * "diag_approx" then fail with status -1.  The "_profile" bivariate
  * kernels return the profile likelihood and set variance1 / variance2
  * (MLE_dexact.c:403-421).  With a non-empty recovery_file, an evaluation
  * whose iteration number has a line there returns that line's likelihood
