@@ -328,7 +328,7 @@ def run_kriging(steps: int) -> None:
     print(json.dumps(out), flush=True)
 
 
-C4_FITS = ("lbfgsb", "lbfgsb_mixed", "bobyqa_ftol1e-5", "bobyqa_ftol1e-7")
+C4_FITS = ("lbfgsb", "lbfgsb_mixed", "lbfgsb_mixed_then_fp64", "bobyqa_ftol1e-5", "bobyqa_ftol1e-7")
 # the mixed path's options for the gradient-based fit: the benchmark's tile
 # classes, the solve refined to 1e-12
 C4_MIXED = dict(MIXED, ir_steps=8, ir_tol=1e-12)
@@ -397,6 +397,7 @@ def run_c4(steps: int, max_evals: int = 3000, which=C4_FITS) -> None:
     lb, ub = [0.01] * 4, [5.0] * 4
     fits = {}
     for name, meth, st, tol in [("lbfgsb", "lbfgsb", start, 5), ("lbfgsb_mixed", "lbfgsb", start, 5),
+                                ("lbfgsb_mixed_then_fp64", "lbfgsb_mixed", start, 5),
                                 ("bobyqa_ftol1e-5", "bobyqa", None, 5), ("bobyqa_ftol1e-7", "bobyqa", None, 7),
                                 ("bobyqa_ftol1e-9", "bobyqa", None, 9)]:
         if name not in which:
@@ -409,7 +410,7 @@ def run_c4(steps: int, max_evals: int = 3000, which=C4_FITS) -> None:
         print(f"bench: c4 {name}: {r.wall_s:.1f}s {r.evals} evals theta={np.round(r.theta, 6)} l={r.loglik:.6f}",
               file=sys.stderr, flush=True)
     m.close()
-    for k in ("lbfgsb_mixed", "bobyqa_ftol1e-5", "bobyqa_ftol1e-7", "bobyqa_ftol1e-9"):
+    for k in ("lbfgsb_mixed", "lbfgsb_mixed_then_fp64", "bobyqa_ftol1e-5", "bobyqa_ftol1e-7", "bobyqa_ftol1e-9"):
         if k not in fits or "lbfgsb" not in fits:
             continue
         a = np.array(fits["lbfgsb"]["theta"])

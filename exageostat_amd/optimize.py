@@ -63,6 +63,11 @@ def fit(m, lb, ub, start=None, method: str = "lbfgsb", opt_tol: int = 5, max_eva
     method "lbfgsb": L-BFGS-B with the adjoint gradient, stops when the
     relative decrease of -l falls below 10^-opt_tol / |l| (the reference's
     absolute ftol scaled by l) or the projected gradient below gtol * n.
+    method "lbfgsb_mixed": L-BFGS-B over the mixed-precision likelihood and
+    gradient (precision 2 under the model's set_mixed options) until the
+    relative decrease of -l falls below 10^-min(opt_tol, 3) / |l| (above
+    the mixed likelihood's own rounding), then "lbfgsb" over the FP64 ones
+    from that point; theta hat and l come from the FP64 stage.
     start defaults to lb, as in the reference.
     log_params: optimise over log(theta) (all bounds must be positive); the
     covariance parameters span orders of magnitude (variance ~1, range
@@ -70,6 +75,21 @@ def fit(m, lb, ub, start=None, method: str = "lbfgsb", opt_tol: int = 5, max_eva
     a log scale.  Default: on for "lbfgsb", off for "dfo" (the reference's
     BOBYQA works on theta itself).
     """
+    if method == "lbfgsb_mixed":
+        prec = m.precision
+        try:
+            m.precision = 2
+            r1 = fit(m, lb, ub, start, "lbfgsb", opt_tol=min(opt_tol, 3), max_evals=max_evals, gtol=gtol,
+                     log_params=log_params)
+            m.precision = 0
+            r2 = fit(m, lb, ub, r1.theta, "lbfgsb", opt_tol=opt_tol, max_evals=max(1, max_evals - r1.evals),
+                     gtol=gtol, log_params=log_params)
+        finally:
+            m.precision = prec
+        return FitResult(theta=r2.theta, loglik=r2.loglik, evals=r1.evals + r2.evals,
+                         grad_evals=r1.grad_evals + r2.grad_evals, wall_s=r1.wall_s + r2.wall_s, method=method,
+                         message=f"mixed stage ({r1.evals} evaluations): {r1.message}; FP64 stage "
+                                 f"({r2.evals}): {r2.message}", history=r1.history + r2.history)
     if log_params is None:
         log_params = method == "lbfgsb"
     lb = np.asarray(lb, dtype=float)

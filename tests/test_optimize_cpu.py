@@ -133,3 +133,42 @@ def test_bobyqa_reaches_the_gradient_method_theta():
     assert np.max(np.abs(r_b.theta - r_g.theta) / np.abs(r_g.theta)) <= 1e-3, (r_b.theta, r_g.theta)
     assert r_b.loglik >= r_g.loglik - 1e-6 * abs(r_g.loglik)
     assert r_g.evals < r_b.evals
+
+
+class NoisyPrecisionMLE(OracleMLE):
+    """OracleMLE with a precision switch: at precision 2 l carries a small
+    deterministic perturbation (standing in for the mixed path's rounding),
+    and the precision used by each evaluation is recorded."""
+
+    def __init__(self, x, y, z):
+        super().__init__(x, y, z)
+        self.precision = 0
+        self.calls = []
+
+    def loglik(self, theta, grad=False):
+        self.calls.append(self.precision)
+        out = super().loglik(theta, grad)
+        if self.precision != 2:
+            return out
+        bump = 1e-7 * np.sin(1e3 * float(np.sum(theta)))
+        return (out[0] + bump, out[1]) if grad else out + bump
+
+
+def test_lbfgsb_mixed_then_fp64():
+    """method "lbfgsb_mixed": a precision-2 stage, then an FP64 stage from
+    its point; theta hat is the FP64 maximum (that of plain "lbfgsb" within
+    1e-4), the model's precision is restored, and both stages ran."""
+    base = _problem()
+    m = NoisyPrecisionMLE(base.x, base.y, base.z)
+    lb, ub = [0.01, 0.01, 0.01], [5.0, 5.0, 5.0]
+    start = [0.5, 0.05, 1.0]
+    ref = optimize.fit(m, lb, ub, start=start, method="lbfgsb", opt_tol=7)
+    m.calls.clear()
+    r = optimize.fit(m, lb, ub, start=start, method="lbfgsb_mixed", opt_tol=7)
+    assert m.precision == 0
+    assert 2 in m.calls and m.calls[-1] == 0
+    first_fp64 = m.calls.index(0)
+    assert all(p == 2 for p in m.calls[:first_fp64]) and all(p == 0 for p in m.calls[first_fp64:])
+    assert r.evals == len(m.calls)
+    assert np.max(np.abs(r.theta - ref.theta) / np.abs(ref.theta)) <= 1e-4, (r.theta, ref.theta)
+    assert abs(r.loglik - ref.loglik) <= 1e-9 * abs(ref.loglik)
