@@ -17,15 +17,17 @@ namespace exa {
 // U (may be null): a lower tile matrix of A's grid that receives Z_mk^T, so
 // that the sweep reads no transposed operands (the memory of a second
 // lower-triangular matrix for a faster sweep).
+// f32 (with U): the column stage on the FP32 matrix cores (operands
+// rounded to FP32, FP32 sums; the trmm and diagonal stages stay FP64).
 void tile_potri_inplace(TileMatrix& A, CholWorkspace& ws, double* P, double* Q, hipStream_t s, double* W = nullptr,
-                        int max_split = 1, TileMatrix* U = nullptr);
+                        int max_split = 1, TileMatrix* U = nullptr, bool f32 = false);
 
 double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric, double* grad,
                            double* logdet_out, double* dotp_out, int* info);
 // l from the mixed-precision path and its gradient with alpha = the refined
 // Sigma^-1 z and the trace term from the Takahashi sweep of the mixed factor
 // (converted to FP64 in place of the FP64 Sigma, which a lean context does
-// not hold).
+// not hold; the sweep's column stage on the FP32 matrix cores).
 double mixed_adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric,
                                  const MixedOptions& opt, double* grad, double* logdet_out, double* dotp_out,
                                  int* info, MixedReport* rep);

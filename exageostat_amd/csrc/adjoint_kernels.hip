@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(C::NT) k_trmm_panel(TileView v, int k, const d
 // Persistent launch, as in the Cholesky update: work-group w runs on XCD
 // w % nx and the XCDs take contiguous slices of the unit list (units of one
 // range s and row m share their A tiles and every P_j of the range).
-template <class C, bool HAS_U>
+template <class C, bool HAS_U, bool F32>
 __device__ __forceinline__ void takahashi_unit(const TileView& v, int k, const double* P, double* W, int nsplit,
                                                const TileView& U, long u, double* lds) {
   const int qm = v.ts / C::BM, qn = v.ts / C::BN;
@@ -76,6 +76,21 @@ __device__ __forceinline__ void takahashi_unit(const TileView& v, int k, const d
   const int chunk = (int)((below + nsplit - 1) / nsplit);
   const int j_lo = k + 1 + sp * chunk;
   const int j_hi = j_lo + chunk < v.mt ? j_lo + chunk : v.mt;
+  if constexpr (F32) {
+    // FP32 matrix cores: operands rounded to FP32 in LDS, FP32 sums
+    f32x4 acc[C::FM][C::FN];
+    zero_acc32<C>(acc);
+#pragma unroll 1
+    for (int j = j_lo; j < j_hi; ++j) {
+      const double* B = P + (long)j * ts * ts + bj * C::BN;
+      const double* A = (j <= m ? tile_f64(v, m, j) : tile_f64(U, j, m)) + bi * C::BM;
+      gemm_nt_acc32<C>(acc, A, ts, B, ts, (int)ts, reinterpret_cast<float*>(lds), -1.f);
+    }
+    double* out = nsplit > 1 ? W + ((long)sp * below + (m - k - 1)) * ts * ts : tile_f64(v, m, k);
+    acc32_store<C>(acc, out + bi * C::BM + (long)bj * C::BN * ts, ts);
+    if (nsplit == 1) acc32_store_t<C>(acc, tile_f64(U, m, k) + bj * C::BN + (long)bi * C::BM * ts, ts);
+    return;
+  }
   dx4 acc[C::FM][C::FN];
   zero_acc<C>(acc);
 #pragma unroll 1
@@ -98,10 +113,12 @@ __device__ __forceinline__ void takahashi_unit(const TileView& v, int k, const d
     if (nsplit == 1) acc_store_t<C>(acc, tile_f64(U, m, k) + bj * C::BN + (long)bi * C::BM * ts, ts);
 }
 
-template <class C, bool HAS_U>
+template <class C, bool HAS_U, bool F32 = false>
 __global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, const double* P, double* W, int nsplit,
                                                          TileView U, long units, int nx) {
-  __shared__ __attribute__((aligned(16))) double lds[HAS_U ? RingLds<C>::DOUBLES : C::LDS_DOUBLES];
+  static_assert(!F32 || HAS_U, "the FP32 column stage reads the transposed copy");
+  __shared__ __attribute__((aligned(16))) double lds[F32 ? C::LDS_DOUBLES / 2
+                                                         : (HAS_U ? RingLds<C>::DOUBLES : C::LDS_DOUBLES)];
   const int w = blockIdx.x;
   const int x = w % nx;
   const int gx = gridDim.x / nx;
@@ -109,7 +126,7 @@ __global__ void __launch_bounds__(C::NT) k_takahashi_col(TileView v, int k, cons
   const long start = (long)x * per;
   const long end = start + per < units ? start + per : units;
 #pragma unroll 1
-  for (long u = start + w / nx; u < end; u += gx) takahashi_unit<C, HAS_U>(v, k, P, W, nsplit, U, u, lds);
+  for (long u = start + w / nx; u < end; u += gx) takahashi_unit<C, HAS_U, F32>(v, k, P, W, nsplit, U, u, lds);
 }
 
 // Z_mk = sum_s W[s][m - k - 1] for m in (k, mt), s ascending, and (with
@@ -184,14 +201,14 @@ __global__ void k_sum_tiles(const double* Q, int cnt, long len, double* out) {
 // one tile per range, and only as many as the slab buffer holds) whose work
 // units fill whole rounds of the concurrent work-groups best; each extra
 // range costs a write and a read of its slab, counted as 2% per range.
-template <class C>
+template <class C, bool F32 = false>
 static int takahashi_splits(const TileView& v, int k, int max_split, int* slots_out) {
   static int slots = 0;
   if (!slots) {
     int dev = 0, cus = 0, occ = 1;
     EXA_HIP(hipGetDevice(&dev));
     EXA_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    EXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_takahashi_col<C, true>, C::NT, 0));
+    EXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_takahashi_col<C, true, F32>, C::NT, 0));
     slots = cus * (occ > 0 ? occ : 1);
     if (slots >= 8) slots -= slots % 8;
   }
@@ -215,14 +232,16 @@ static int takahashi_splits(const TileView& v, int k, int max_split, int* slots_
 
 template <class C>
 static void takahashi_cfg(const TileView& v, int k, const double* X, double* P, double* Q, double* W, int max_split,
-                          const TileView& U, int stage, hipStream_t s) {
+                          const TileView& U, int stage, hipStream_t s, bool f32) {
   const long per = (long)(v.ts / C::BM) * (v.ts / C::BN);
   const long below = v.mt - k - 1;
   if (stage == 0 && below > 0)
     hipLaunchKernelGGL(k_trmm_panel<C>, dim3((unsigned)(below * per)), dim3(C::NT), 0, s, v, k, X, P);
   if (stage == 1 && below > 0) {
     int slots = 0;
-    const int ns = takahashi_splits<C>(v, k, W ? max_split : 1, &slots);
+    const bool f32u = f32 && U.ptr;
+    const int ns = f32u ? takahashi_splits<C, true>(v, k, W ? max_split : 1, &slots)
+                        : takahashi_splits<C>(v, k, W ? max_split : 1, &slots);
     const long units = below * per * ns;
     // persistent grid: at most one round of resident work-groups, a
     // multiple of the 8 XCDs when it is that large
@@ -230,7 +249,10 @@ static void takahashi_cfg(const TileView& v, int k, const double* X, double* P, 
     int nx = 8;
     if (grid >= 8) grid -= grid % 8;
     else nx = 1;
-    if (U.ptr)
+    if (f32u)
+      hipLaunchKernelGGL((k_takahashi_col<C, true, true>), dim3((unsigned)grid), dim3(C::NT), 0, s, v, k,
+                         (const double*)P, W, ns, U, units, nx);
+    else if (U.ptr)
       hipLaunchKernelGGL((k_takahashi_col<C, true>), dim3((unsigned)grid), dim3(C::NT), 0, s, v, k,
                          (const double*)P, W, ns, U, units, nx);
     else
@@ -249,12 +271,12 @@ static void takahashi_cfg(const TileView& v, int k, const double* X, double* P, 
 }
 
 void launch_takahashi_stage(const TileView& v, int k, const double* X, double* P, double* Q, double* Zkk,
-                            int stage, hipStream_t s, double* W, int max_split, const TileView* U) {
+                            int stage, hipStream_t s, double* W, int max_split, const TileView* U, bool f32) {
   TileView u{};
   if (U) u = *U;
-  if (v.ts % 192 == 0) takahashi_cfg<C192>(v, k, X, P, Q, W, max_split, u, stage, s);
-  else if (v.ts % 128 == 0) takahashi_cfg<C128>(v, k, X, P, Q, W, max_split, u, stage, s);
-  else takahashi_cfg<CT64>(v, k, X, P, Q, W, max_split, u, stage, s);
+  if (v.ts % 192 == 0) takahashi_cfg<C192>(v, k, X, P, Q, W, max_split, u, stage, s, f32);
+  else if (v.ts % 128 == 0) takahashi_cfg<C128>(v, k, X, P, Q, W, max_split, u, stage, s, f32);
+  else takahashi_cfg<CT64>(v, k, X, P, Q, W, max_split, u, stage, s, f32);
   if (stage == 2) {
     const long len = (long)v.ts * v.ts;
     hipLaunchKernelGGL(k_sum_tiles, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, (const double*)Q,

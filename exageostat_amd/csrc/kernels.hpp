@@ -159,10 +159,12 @@ struct MaternGradParams {
 // W (may be null): slabs for splitting stage 1's sum over j into up to
 // max_split ranges, max_split * (mt - 1) tiles.  U (may be null): a lower
 // tile matrix of v's geometry that receives Z_mk^T in tile (m, k), so that
-// the later stages read no transposed operands.
+// the later stages read no transposed operands.  f32 (with U): stage 1 on
+// the FP32 matrix cores (operands rounded to FP32, FP32 sums).
 void launch_takahashi_stage(const TileView& v, int k, const double* X, double* P, double* Q, double* Zkk,
                             int stage, hipStream_t s, double* W = nullptr, int max_split = 1,
-                            const TileView* U = nullptr);
+                            const TileView* U = nullptr,
+                            bool f32 = false);
 struct BivariateGradParams {
   const double* x;
   const double* y;

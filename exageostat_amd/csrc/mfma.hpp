@@ -380,4 +380,135 @@ __device__ __forceinline__ void acc_load(dx4 (&acc)[C::FM][C::FN], const T* Cp, 
     }
 }
 
+// ---------------------------------------------------- FP32 matrix cores --
+// v_mfma_f32_16x16x4f32: operands as for the FP64 instruction (lane l
+// supplies A[l%16][l/16] and B[l/16][l%16]); result: lane l, element v holds
+// D[4*(l/16)+v][l%16].
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <class C>
+__device__ __forceinline__ void zero_acc32(f32x4 (&acc)[C::FM][C::FN]) {
+#pragma unroll
+  for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < C::FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+}
+
+// acc += sign * A * B^T with FP64 operands rounded to FP32 when staged into
+// LDS (the register-staged double buffering of gemm_nt_acc, plain operand
+// layout) and FP32 matrix cores at twice the FP64 rate, FP32 accumulation.
+// lds: C::LDS_DOUBLES doubles, used as twice as many floats' worth of half.
+template <class C>
+__device__ __forceinline__ void gemm_nt_acc32(f32x4 (&acc)[C::FM][C::FN], const double* __restrict__ A, long lda,
+                                              const double* __restrict__ B, long ldb, int K, float* lds,
+                                              float sign) {
+  if (K <= 0) return;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave / C::WGN;
+  const int wn = wave - wm * C::WGN;
+  float* As = lds;
+  float* Bs = lds + 2 * C::A_STAGE;
+  double2 ra[C::A_LOADS];
+  double2 rb[C::B_LOADS];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int t = 0; t < C::A_LOADS; ++t) {
+      const int e = tid + t * C::NT;
+      const int c = e / (C::BM / 2);
+      const int r2 = e - c * (C::BM / 2);
+      ra[t] = ld_global2(A + (long)(k0 + c) * lda + 2 * r2);
+    }
+#pragma unroll
+    for (int t = 0; t < C::B_LOADS; ++t) {
+      const int e = tid + t * C::NT;
+      const int c = e / (C::BN / 2);
+      const int r2 = e - c * (C::BN / 2);
+      rb[t] = ld_global2(B + (long)(k0 + c) * ldb + 2 * r2);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < C::A_LOADS; ++t) {
+      const int e = tid + t * C::NT;
+      const int c = e / (C::BM / 2);
+      const int r2 = e - c * (C::BM / 2);
+      *reinterpret_cast<float2*>(As + buf * C::A_STAGE + c * C::LDA + 2 * r2) =
+          float2{(float)ra[t].x, (float)ra[t].y};
+    }
+#pragma unroll
+    for (int t = 0; t < C::B_LOADS; ++t) {
+      const int e = tid + t * C::NT;
+      const int c = e / (C::BN / 2);
+      const int r2 = e - c * (C::BN / 2);
+      *reinterpret_cast<float2*>(Bs + buf * C::B_STAGE + c * C::LDB + 2 * r2) =
+          float2{(float)rb[t].x, (float)rb[t].y};
+    }
+  };
+  const int nk = K / C::BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int kc = 0; kc < nk; ++kc) {
+    const int buf = kc & 1;
+    if (kc + 1 < nk) gload((kc + 1) * C::BK);
+    const float* Ab = As + buf * C::A_STAGE + wm * C::WM + (lane & 15);
+    const float* Bb = Bs + buf * C::B_STAGE + wn * C::WN + (lane & 15);
+#pragma unroll
+    for (int kk = 0; kk < C::BK; kk += 4) {
+      const int krow = kk + (lane >> 4);
+      float a[C::FM], b[C::FN];
+#pragma unroll
+      for (int i = 0; i < C::FM; ++i) a[i] = sign * Ab[krow * C::LDA + i * 16];
+#pragma unroll
+      for (int j = 0; j < C::FN; ++j) b[j] = Bb[krow * C::LDB + j * 16];
+#pragma unroll
+      for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+        for (int j = 0; j < C::FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (kc + 1 < nk) sstore(buf ^ 1);
+    __syncthreads();
+  }
+}
+
+// Cp[r + c*ldc] = acc(r, c) as FP64 (the FP32 result layout).
+template <class C>
+__device__ __forceinline__ void acc32_store(const f32x4 (&acc)[C::FM][C::FN], double* Cp, long ldc) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave / C::WGN;
+  const int wn = wave - wm * C::WGN;
+  typedef __attribute__((address_space(1))) double gD;
+  gD* G = (gD*)(uintptr_t)(Cp + wm * C::WM + 4 * (lane >> 4) + (long)(wn * C::WN + (lane & 15)) * ldc);
+#pragma unroll
+  for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+    for (int j = 0; j < C::FN; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) G[i * 16 + v + (long)j * 16 * ldc] = (double)acc[i][j][v];
+}
+
+// Cp[c + r*ldc] = acc(r, c) as FP64: the block stored transposed.
+template <class C>
+__device__ __forceinline__ void acc32_store_t(const f32x4 (&acc)[C::FM][C::FN], double* Cp, long ldc) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave / C::WGN;
+  const int wn = wave - wm * C::WGN;
+  typedef __attribute__((address_space(1))) double gD;
+  gD* G = (gD*)(uintptr_t)(Cp + wn * C::WN + (lane & 15) + (long)(wm * C::WM + 4 * (lane >> 4)) * ldc);
+#pragma unroll
+  for (int i = 0; i < C::FM; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const long ro = (long)(i * 16 + v) * ldc;
+#pragma unroll
+      for (int j = 0; j < C::FN; ++j) G[ro + j * 16] = (double)acc[i][j][v];
+    }
+}
+
 }  // namespace exa
