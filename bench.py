@@ -221,8 +221,8 @@ def run_c5(steps: int) -> None:
     """Parsimonious bivariate Matern, 50,000 locations x 2 variables
     (n = 100,000 rows; 224 x 224 jittered grid truncated), theta = (1, 1, 0.1,
     0.5, 1.0, 0.5): FP64 vs mixed l, the six-parameter adjoint gradient with
-    central differences (h = 1e-6 relative) for two of its components, and
-    kriging of 2,000 held-out locations per variable (seeded random split of
+    central differences (h = 1e-6 relative) for two of its components, the
+    same gradient on the mixed path against the FP64 one, and kriging of 2,000 held-out locations per variable (seeded random split of
     the locations; both sets keep the Morton order)."""
     import exageostat_amd
     from exageostat_amd import mle
@@ -249,8 +249,25 @@ def run_c5(steps: int) -> None:
         fd[p] = (m.loglik(tp) - m.loglik(tm)) / (2 * h)
     out["grad_fd_check"] = {str(p): {"adjoint": g[p], "central_fd": v, "rel": abs(g[p] - v) / abs(v)}
                             for p, v in fd.items()}
-    m.close()
     print(f"bench: gradient {out['grad_s']:.1f}s", file=sys.stderr, flush=True)
+    # the six-parameter gradient on the mixed path (alpha from the refined
+    # solve, the Takahashi sweep of the mixed factor), with the option set
+    # that holds l to 1e-6 here: the sweep in FP64, then its column stage on
+    # the FP32 matrix cores
+    l64 = out["fp64_vs_mixed"]["fp64"]["loglik"]
+    m.precision = 2
+    for name, f32 in (("mixed_grad", False), ("mixed_grad_f32_sweep", True)):
+        opts = dict(VARIANTS_C5["mixed_lolo_nobf16_fp64corr_0_3"], grad_f32=f32)
+        m.set_mixed(**opts)
+        t0 = time.perf_counter()
+        lm, gm = m.loglik(theta, grad=True)
+        out[name] = {"s": time.perf_counter() - t0, "options": opts, "status": m.status, "grad": list(gm),
+                     "max_rel_diff_vs_fp64": float(np.max(np.abs(np.asarray(gm) - np.asarray(g)))
+                                                   / np.max(np.abs(g))),
+                     "rel_dl_vs_fp64": abs(lm - l64) / abs(l64)}
+        print(f"bench: {name} {out[name]['s']:.1f}s", file=sys.stderr, flush=True)
+    m.precision = 0
+    m.close()
     # kriging: 2,000 held-out locations (both variables)
     zz = z.reshape(nloc, 2)
     perm = np.random.default_rng(0).permutation(nloc)
@@ -330,8 +347,10 @@ def run_kriging(steps: int) -> None:
 
 C4_FITS = ("lbfgsb", "lbfgsb_mixed", "lbfgsb_mixed_then_fp64", "bobyqa_ftol1e-5", "bobyqa_ftol1e-7")
 # the mixed path's options for the gradient-based fit: the benchmark's tile
-# classes, the solve refined to 1e-12
-C4_MIXED = dict(MIXED, ir_steps=8, ir_tol=1e-12)
+# classes, the solve refined to 1e-12, the Takahashi sweep's column stage on
+# the FP32 matrix cores (with the nugget, Sigma is well enough conditioned
+# for its rounding to stay below the mixed factor's own error)
+C4_MIXED = dict(MIXED, ir_steps=8, ir_tol=1e-12, grad_f32=True)
 
 
 def run_c4(steps: int, max_evals: int = 3000, which=C4_FITS) -> None:

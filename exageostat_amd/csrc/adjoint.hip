@@ -244,13 +244,15 @@ double mixed_adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta
     return ll;
   }
   // The FP64 Sigma has served the refinement; it now receives the mixed
-  // factor, whose Takahashi sweep gives the trace term, with its column
-  // stage on the FP32 matrix cores (the factor's own error is larger than
-  // FP32 rounding; 0.78 against 1.05 s for l + gradient at n = 40,000, the
-  // gradient unchanged in accuracy).  ctx.work holds the refined
-  // Sigma^-1 z.
+  // factor, whose Takahashi sweep gives the trace term.  With opt.grad_f32
+  // the sweep's column stage runs on the FP32 matrix cores: 0.74 against
+  // 1.01 s for l + gradient at n = 40,000 with a 0.01 nugget, the gradient
+  // as accurate as with the FP64 sweep there; on a worse-conditioned Sigma
+  // (the parsimonious bivariate kernel at 50,000 locations) its rounding
+  // error is 3.7e-3 of the largest component against 1.8e-4 from the mixed
+  // factor alone.  ctx.work holds the refined Sigma^-1 z.
   mixed_factor_to_fp64(ctx, A, ctx.s1);
-  gradient_from_factor(ctx, kid, theta, metric, A, true, grad, true);
+  gradient_from_factor(ctx, kid, theta, metric, A, true, grad, opt.grad_f32);
   const double t1 = wall_seconds();
   ctx.last = lt;
   ctx.last.grad = t1 - t0 - lt.total;

@@ -27,7 +27,8 @@ double adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int 
 // l from the mixed-precision path and its gradient with alpha = the refined
 // Sigma^-1 z and the trace term from the Takahashi sweep of the mixed factor
 // (converted to FP64 in place of the FP64 Sigma, which a lean context does
-// not hold; the sweep's column stage on the FP32 matrix cores).
+// not hold; the sweep's column stage on the FP32 matrix cores when
+// opt.grad_f32 is set, in FP64 otherwise).
 double mixed_adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta, int metric,
                                  const MixedOptions& opt, double* grad, double* logdet_out, double* dotp_out,
                                  int* info, MixedReport* rep);

@@ -376,6 +376,7 @@ MixedOptions mixed_options_of(const MLE_data* data, bool single) {
   opt.ir_tol = data->mp_ir_tol;
   opt.diag_thick = data->diag_thick;
   opt.lolo = data->mp_lolo != 0;
+  opt.grad_f32 = data->mp_grad_f32 != 0;
   return opt;
 }
 

@@ -17,6 +17,9 @@ struct MixedOptions {
   double ir_tol = 0;
   int diag_thick = 1;
   bool lolo = false;  // FP32 class: hi*hi + hi*lo + lo*hi (+ lo*lo when set)
+  // gradient: the Takahashi sweep's column stage on the FP32 matrix cores
+  // instead of FP64 (its error grows with the condition number of Sigma)
+  bool grad_f32 = false;
 };
 
 // What the refinement did in the last mixed evaluation.

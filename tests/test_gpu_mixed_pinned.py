@@ -126,17 +126,20 @@ def test_refinement_stopping_rule(lib):
     m.close()
 
 
+@pytest.mark.parametrize("grad_f32", [False, True], ids=["fp64_sweep", "fp32_sweep"])
 @pytest.mark.parametrize("n,ts", [(1600, 320), (5760, 192)], ids=["per_class", "fused"])
-def test_mixed_gradient_matches_oracle(lib, n, ts):
+def test_mixed_gradient_matches_oracle(lib, n, ts, grad_f32):
     """precision 2 with a gradient request: alpha from the refined solve,
-    the trace term from the Takahashi sweep of the mixed factor; every
-    component against the oracle's dense FP64 gradient, relative to the
-    gradient's largest component."""
+    the trace term from the Takahashi sweep of the mixed factor (its column
+    stage in FP64, or on the FP32 matrix cores with grad_f32: 64 x 64 blocks
+    at ts = 320, 192 x 192 at ts = 192); every component against the
+    oracle's dense FP64 gradient, relative to the gradient's largest
+    component."""
     x, y, z, (ll, _, _) = _field(5760, THETAS[0])
     x, y, z = x[:n], y[:n], z[:n]
     l_ref, g_ref = likelihood.gradient(K, x, y, z, np.array(THETAS[0]))
     m = lib.ExactMLE(x, y, ts, precision=2)
-    m.set_mixed(**dict(OPTS, ir_steps=8, ir_tol=1e-12))
+    m.set_mixed(**dict(OPTS, ir_steps=8, ir_tol=1e-12, grad_f32=grad_f32))
     m.set_z(z)
     val, g = m.loglik(np.array(THETAS[0]), grad=True)
     status = m.status
@@ -144,10 +147,37 @@ def test_mixed_gradient_matches_oracle(lib, n, ts):
     val2 = m.loglik(np.array(THETAS[0]))
     m.close()
     err = np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref))
-    print(f"mixed gradient n={n} ts={ts}: rel err {err:.2e}, g={g}, oracle={g_ref}", flush=True)
+    print(f"mixed gradient n={n} ts={ts} grad_f32={grad_f32}: rel err {err:.2e}, g={g}, oracle={g_ref}",
+          flush=True)
     assert status == 0
     assert abs(val - l_ref) <= MIXED_TOL * abs(l_ref) and abs(val2 - l_ref) <= MIXED_TOL * abs(l_ref)
     assert err <= GRAD_TOL, err
+
+
+@pytest.mark.parametrize("grad_f32", [False, True], ids=["fp64_sweep", "fp32_sweep"])
+def test_mixed_gradient_bivariate_matches_oracle(lib, grad_f32):
+    """precision 2 with a gradient request for the parsimonious bivariate
+    kernel, all six parameters (n = 800 rows, ts = 128: per-class updates
+    with 128 x 128 blocks, a partial last tile, and with grad_f32 the
+    sweep's FP32 column stage on the same blocks), against the oracle's
+    dense FP64 gradient, at the fixture's theta and at one with a negative
+    rho."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "bivariate_400.npz"))
+    kb = "bivariate_matern_parsimonious"
+    for theta in (np.array(g["theta"], dtype=float), np.array([0.8, 1.2, 0.12, 0.7, 1.3, -0.4])):
+        l_ref, g_ref = likelihood.gradient(kb, g["x"], g["y"], g["z"], theta)
+        m = lib.ExactMLE(g["x"], g["y"], 128, kernel=kb, precision=2)
+        m.set_mixed(**dict(OPTS, ir_steps=8, ir_tol=1e-12, grad_f32=grad_f32))
+        m.set_z(g["z"])
+        val, gr = m.loglik(theta, grad=True)
+        status = m.status
+        m.close()
+        err = np.max(np.abs(gr - g_ref)) / np.max(np.abs(g_ref))
+        print(f"mixed bivariate gradient theta={theta} grad_f32={grad_f32}: rel err {err:.2e}", flush=True)
+        assert status == 0 and len(gr) == 6
+        assert abs(val - l_ref) <= MIXED_TOL * abs(l_ref)
+        assert err <= GRAD_TOL, (theta, gr, g_ref)
 
 
 def test_single_precision_gradient_is_refused(lib):
