@@ -262,6 +262,7 @@ def run_c5(steps: int) -> None:
         t0 = time.perf_counter()
         lm, gm = m.loglik(theta, grad=True)
         out[name] = {"s": time.perf_counter() - t0, "options": opts, "status": m.status, "grad": list(gm),
+                     "sweep_residual": m.data.mp_grad_check,
                      "max_rel_diff_vs_fp64": float(np.max(np.abs(np.asarray(gm) - np.asarray(g)))
                                                    / np.max(np.abs(g))),
                      "rel_dl_vs_fp64": abs(lm - l64) / abs(l64)}
@@ -348,9 +349,10 @@ def run_kriging(steps: int) -> None:
 C4_FITS = ("lbfgsb", "lbfgsb_mixed", "lbfgsb_mixed_then_fp64", "bobyqa_ftol1e-5", "bobyqa_ftol1e-7")
 # the mixed path's options for the gradient-based fit: the benchmark's tile
 # classes, the solve refined to 1e-12, the Takahashi sweep's column stage on
-# the FP32 matrix cores (with the nugget, Sigma is well enough conditioned
-# for its rounding to stay below the mixed factor's own error)
-C4_MIXED = dict(MIXED, ir_steps=8, ir_tol=1e-12, grad_f32=True)
+# the FP32 matrix cores unless its residual along z exceeds 1e-6 (then the
+# gradient is recomputed with the FP64 sweep; with the nugget, Sigma is well
+# enough conditioned for the FP32 sweep to pass: residual 4e-7 at theta0)
+C4_MIXED = dict(MIXED, ir_steps=8, ir_tol=1e-12, grad_f32=2)
 
 
 def run_c4(steps: int, max_evals: int = 3000, which=C4_FITS) -> None:
@@ -403,7 +405,7 @@ def run_c4(steps: int, max_evals: int = 3000, which=C4_FITS) -> None:
     for _ in range(steps):
         lm, gm = m.loglik(start, grad=True)
     mixed = {"loglik_grad_s": (time.perf_counter() - t0) / steps, "options": C4_MIXED, "status": m.status,
-             "rel_dl_vs_fp64": abs(lm - l0) / abs(l0),
+             "sweep_residual": m.data.mp_grad_check, "rel_dl_vs_fp64": abs(lm - l0) / abs(l0),
              "grad_max_rel_diff_vs_fp64": float(np.max(np.abs(gm - g0)) / np.max(np.abs(g0)))}
     t0 = time.perf_counter()
     for _ in range(steps):

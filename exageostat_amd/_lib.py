@@ -76,7 +76,7 @@ class MLEData(ctypes.Structure):
         ("time_gen", _D), ("time_facto", _D), ("time_logdet", _D), ("time_solve", _D), ("time_total", _D),
         ("mp_fp32_corr", _D), ("mp_bf16", _I), ("mp_ir_steps", _I), ("mp_fp64_corr", _D), ("mp_lolo", _I),
         ("mp_ir_tol", _D), ("mp_ir_done", _I), ("mp_ir_change", _D), ("mp_lean", _I),
-        ("mp_grad_f32", _I),
+        ("mp_grad_f32", _I), ("mp_grad_check", _D),
     ]
 
 

@@ -250,9 +250,25 @@ double mixed_adjoint_loglik_grad(Context& ctx, KernelId kid, const double* theta
   // as accurate as with the FP64 sweep there; on a worse-conditioned Sigma
   // (the parsimonious bivariate kernel at 50,000 locations) its rounding
   // error is 3.7e-3 of the largest component against 1.8e-4 from the mixed
-  // factor alone.  ctx.work holds the refined Sigma^-1 z.
+  // factor alone.  The swept inverse is then checked along z against the
+  // triangular solves with the factor it inverted; with opt.grad_f32 == 2 a
+  // residual above kSweepResidualMax recomputes the gradient with the FP64
+  // sweep.  ctx.work holds the refined Sigma^-1 z.
   mixed_factor_to_fp64(ctx, A, ctx.s1);
-  gradient_from_factor(ctx, kid, theta, metric, A, true, grad, opt.grad_f32);
+  if (opt.grad_f32) mixed_probe_solve(ctx, A, ctx.s1);
+  gradient_from_factor(ctx, kid, theta, metric, A, true, grad, opt.grad_f32 != 0);
+  if (opt.grad_f32) {
+    const double resid = mixed_probe_residual(ctx, A, ctx.s1);
+    if (opt.grad_f32 == 2 && !(resid <= kSweepResidualMax)) {
+      MixedOptions o64 = opt;
+      o64.grad_f32 = 0;
+      const double ll64 = mixed_adjoint_loglik_grad(ctx, kid, theta, metric, o64, grad, logdet_out, dotp_out, info, rep);
+      if (rep) rep->sweep_residual = resid;
+      ctx.last.total = wall_seconds() - t0;  // both attempts
+      return ll64;
+    }
+    if (rep) rep->sweep_residual = resid;
+  }
   const double t1 = wall_seconds();
   ctx.last = lt;
   ctx.last.grad = t1 - t0 - lt.total;

@@ -376,7 +376,7 @@ MixedOptions mixed_options_of(const MLE_data* data, bool single) {
   opt.ir_tol = data->mp_ir_tol;
   opt.diag_thick = data->diag_thick;
   opt.lolo = data->mp_lolo != 0;
-  opt.grad_f32 = data->mp_grad_f32 != 0;
+  opt.grad_f32 = data->mp_grad_f32;
   return opt;
 }
 
@@ -532,6 +532,7 @@ double evaluate(MLE_data* data, unsigned n, const double* theta, double* grad, P
         else loglik = mixed_loglik(*ctx, kid, theta, metric, opt, &logdet, &dotp, &info, &rep);
         data->mp_ir_done = rep.ir_done;
         data->mp_ir_change = rep.ir_change;
+        data->mp_grad_check = rep.sweep_residual;
         if (path == Path::kSingle) data->sdotp = (float)dotp;
       }
       const long N = ctx->N;

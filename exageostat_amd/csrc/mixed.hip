@@ -349,6 +349,28 @@ void mixed_factor_to_fp64(Context& ctx, TileMatrix& dst, hipStream_t s) {
   ctx.ws.factored = dst.tile_raw(0, 0);
 }
 
+// x0 = F^-T F^-1 z with F the widened mixed factor (before its Takahashi
+// sweep), kept in the refinement's correction vector.
+void mixed_probe_solve(Context& ctx, TileMatrix& F, hipStream_t s) {
+  MixedState& st = *ctx.mixed;
+  EXA_HIP(hipMemcpyAsync(st.r.d, ctx.Zcpy.d, sizeof(double) * ctx.work.padded, hipMemcpyDeviceToDevice, s));
+  tile_trsv(F, ctx.ws, st.r.d, 0, s);
+  tile_trsv(F, ctx.ws, st.r.d, 1, s);
+}
+
+// ||Z z - x0|| / ||x0||, Z the swept inverse in the lower tiles of Zinv and
+// x0 from mixed_probe_solve: the sweep's error along z, measured against the
+// triangular solves with the factor it inverted.
+double mixed_probe_residual(Context& ctx, TileMatrix& Zinv, hipStream_t s) {
+  MixedState& st = *ctx.mixed;
+  const long N = ctx.N;
+  symv(Zinv, ctx.Zcpy.d, st.x.d, st.part, s);
+  launch_axpby(1.0, st.x.d, -1.0, st.r.d, st.x.d, N, s);
+  const double dd = device_dot(st.x.d, st.x.d, N, ctx.d_partial, s);
+  const double xx = device_dot(st.r.d, st.r.d, N, ctx.d_partial, s);
+  return xx > 0.0 ? std::sqrt(dd / xx) : 0.0;
+}
+
 // Tile precision map of the last mixed evaluation (lower-packed order);
 // returns the number of tiles or 0 when no mixed factor exists.
 long mixed_prec_map(const Context& ctx, uint8_t* out, long cap) {

@@ -183,17 +183,19 @@ class ExactMLE:
 
     def set_mixed(self, fp32_corr: float = 0.0, bf16: bool = False, fp64_corr: float = 0.0,
                   diag_thick: int = 1, ir_steps: int = 3, lolo: bool = False, ir_tol: float = 0.0,
-                  grad_f32: bool = False) -> None:
+                  grad_f32: int = 0) -> None:
         """Mixed-precision options (used when precision == 2): tiles whose
         largest correlation is >= fp64_corr stay FP64, tiles below fp32_corr
         get BF16 updates when bf16 is on, the rest are FP32; the solve is
         refined at most ir_steps times in FP64, stopping early once a step
         changes z^T Sigma^-1 z by at most ir_tol (relative; 0: always
         ir_steps steps).  lolo: the FP32 tiles' split-BF16 products keep
-        their lo*lo term (four products instead of three).  grad_f32: a
-        gradient request runs the Takahashi sweep's column stage on the
-        FP32 matrix cores (faster; its error grows with the condition
-        number of Sigma)."""
+        their lo*lo term (four products instead of three).  grad_f32: 1 (or
+        True) runs a gradient request's Takahashi sweep with its column
+        stage on the FP32 matrix cores (faster; its error grows with the
+        condition number of Sigma) and reports the swept inverse's residual
+        along z in data.mp_grad_check; 2 also recomputes the gradient with
+        the FP64 sweep when that residual exceeds 1e-6."""
         d = self.data
         d.mp_fp32_corr = fp32_corr
         d.mp_bf16 = 1 if bf16 else 0
@@ -202,7 +204,7 @@ class ExactMLE:
         d.mp_ir_steps = ir_steps
         d.mp_lolo = 1 if lolo else 0
         d.mp_ir_tol = ir_tol
-        d.mp_grad_f32 = 1 if grad_f32 else 0
+        d.mp_grad_f32 = int(grad_f32)
 
     @property
     def precision(self) -> int:

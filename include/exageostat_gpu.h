@@ -322,8 +322,14 @@ typedef struct {
                            condition number of Sigma: 4e-8 of the largest
                            gradient component at 40,000 locations with a
                            0.01 nugget, 3.7e-3 (1.8e-4 with 0) for the
-                           bivariate kernel at 50,000 locations.  0 (the
-                           default) runs the sweep in FP64.                 */
+                           bivariate kernel at 50,000 locations.  2 does the
+                           same and, when mp_grad_check exceeds 1e-6,
+                           recomputes the gradient with the FP64 sweep.
+                           0 (the default) runs the sweep in FP64.          */
+  double mp_grad_check; /* mixed, output of mp_grad_f32 1 and 2:
+                           ||Z z - x0|| / ||x0||, Z the FP32-swept inverse
+                           and x0 the solve with the mixed factor; 0 when
+                           the FP32 sweep did not run                       */
 } MLE_data;
 
 /* ---------------------------------------------------------------- setup -- */

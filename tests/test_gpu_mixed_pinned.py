@@ -142,7 +142,7 @@ def test_mixed_gradient_matches_oracle(lib, n, ts, grad_f32):
     m.set_mixed(**dict(OPTS, ir_steps=8, ir_tol=1e-12, grad_f32=grad_f32))
     m.set_z(z)
     val, g = m.loglik(np.array(THETAS[0]), grad=True)
-    status = m.status
+    status, check = m.status, m.data.mp_grad_check
     # the context keeps working after the in-place inverse
     val2 = m.loglik(np.array(THETAS[0]))
     m.close()
@@ -152,6 +152,38 @@ def test_mixed_gradient_matches_oracle(lib, n, ts, grad_f32):
     assert status == 0
     assert abs(val - l_ref) <= MIXED_TOL * abs(l_ref) and abs(val2 - l_ref) <= MIXED_TOL * abs(l_ref)
     assert err <= GRAD_TOL, err
+    # the swept inverse's residual is reported for the FP32 column stage only
+    assert (0.0 < check < 1e-4) if grad_f32 else check == 0.0, check
+
+
+@pytest.mark.parametrize("theta,falls_back", [((1.0, 0.1, 0.5), False), ((1.0, 0.1, 1.5), True)],
+                         ids=["keeps_fp32", "falls_back"])
+def test_mixed_gradient_auto_sweep(lib, theta, falls_back):
+    """grad_f32 = 2: the FP32 column stage's residual along z (Z z against
+    the solve with the mixed factor) decides.  At nu = 0.5 it is 2e-7, below
+    the 1e-6 bound, and the gradient is the FP32 sweep's, bit for bit; at
+    nu = 1.5 (a worse-conditioned Sigma) it is 1e-5, and the gradient is
+    recomputed with the FP64 sweep, bit for bit the grad_f32 = 0 result.
+    Either way the reported residual is the FP32 sweep's."""
+    x, y, _ = datagen.generate_xy_loc(1600, 0)
+    e, _ = datagen.dlarnv(3, [0, 0, 0, 1], 1600)
+    th = np.array(theta)
+    z = likelihood.dzvg(K, x, y, th, e)
+    l_ref, g_ref = likelihood.gradient(K, x, y, z, th)
+    m = lib.ExactMLE(x, y, 320, precision=2)
+    m.set_z(z)
+    out, checks = {}, {}
+    for mode in (0, 1, 2):
+        m.set_mixed(**dict(OPTS, ir_steps=8, ir_tol=1e-12, grad_f32=mode))
+        _, g = m.loglik(th, grad=True)
+        assert m.status == 0
+        out[mode], checks[mode] = np.array(g), m.data.mp_grad_check
+    m.close()
+    errs = {k: np.max(np.abs(v - g_ref)) / np.max(np.abs(g_ref)) for k, v in out.items()}
+    print(f"auto sweep theta={theta}: residual {checks[1]:.2e}, gradient errors {errs}", flush=True)
+    assert checks[0] == 0.0 and checks[2] == checks[1] > 0.0
+    assert (checks[1] > 1e-6) == falls_back
+    assert np.array_equal(out[2], out[0] if falls_back else out[1])
 
 
 @pytest.mark.parametrize("grad_f32", [False, True], ids=["fp64_sweep", "fp32_sweep"])
